@@ -1,0 +1,131 @@
+"""Benchmark: FP64 cell-updates/s of the 2D Euler FV step (HLLC + van Leer MUSCL, SSP-RK2, CFL 0.5).
+
+Workload: the random-Riemann batch -- 64 instances of 512^2, periodic, each seeded by its index,
+four constant quadrants split at a random corner -- advanced in the device-resident loop (per-
+instance dt fused into the stage-2 kernel, RK2 steps captured in a hipGraph).  One "step" advances
+every instance of the batch by one SSP-RK2 step.  Inputs are resident in HBM before timing starts.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-seconds S]
+
+Prints one JSON line.  roofline: the stage-2 kernel (the dominant launch), whose algorithmic HBM
+traffic is 96 B per cell (reads U and U1, writes U^{n+1}); its average duration is measured with
+HIP events on the handle's stream.  cpu_baseline: the reference solver compiled from its sources
+(oracle/_ref) when that build is present, else the repo's CPU port of it, timed on the host cores
+on instance 0 of the batch.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "neural-pde-surrogate_amd"))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+N_INST, N = 64, 512
+HBM_PEAK_GBS = 8000.0
+BYTES_STAGE1, BYTES_STAGE2 = 64, 96  # algorithmic HBM bytes per cell per stage (binary64, 4 variables)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(U0, seconds):
+    import numpy as np
+    import oracle2d
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    g = oracle2d.Grid.make(N, N, bc_x="periodic", bc_y="periodic")
+    if oracle2d.have_reference():
+        kind, runner = "reference", oracle2d.Reference()
+        run = lambda U, k: runner.run(g, "hllc", "vanleer", 0.5, U, 0.5, nout=1, max_steps=k)
+    else:
+        kind, runner = "port", oracle2d.Oracle()
+        run = lambda U, k: runner.run(g, "hllc", "vanleer", 0.5, U, 0.5, nout=1, max_steps=k, threads=threads)
+    steps, elapsed = 4, 0.0
+    while True:
+        U = np.ascontiguousarray(U0.copy())
+        t0 = time.perf_counter()
+        r = run(U, steps)
+        elapsed = time.perf_counter() - t0
+        done = r["steps"]
+        if elapsed >= seconds or steps >= 4096:
+            break
+        steps = int(steps * max(2.0, min(8.0, 1.2 * seconds / max(elapsed, 1e-3))))
+    value = N * N * done / elapsed
+    return {"value": value, "unit": "cell-updates/s", "cores": threads, "kind": kind,
+            "sample": f"instance 0 of the batch (512^2 random Riemann, HLLC + van Leer), {done} SSP-RK2 steps "
+                      f"from t = 0 in {elapsed:.2f} s on {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the batch would shard across GPUs "
+              "by instances with no communication, which this build does not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import npde
+    from npde.families import riemann
+
+    torch.cuda.set_device(0)
+    grids = [npde.make_grid(N, N, bc_x=npde.PERIODIC, bc_y=npde.PERIODIC)] * N_INST
+    ics = [riemann(k, N)[0] for k in range(N_INST)]
+    b = npde.Batch(grids, "hllc", "vanleer", 0.5)
+    for k, U in enumerate(ics):
+        b.set_state(k, U)
+    tuning = b.tuning if args.no_tune else b.autotune(None)
+    for k, U in enumerate(ics):  # autotuning restores the state; set it again anyway
+        b.set_state(k, U)
+    log(f"tuning: {tuning}")
+
+    b.advance_steps(args.warmup, 0.5)
+    s0 = [b.time(k)[1] for k in range(N_INST)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.advance_steps(args.steps, 0.5)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    s1 = [b.time(k)[1] for k in range(N_INST)]
+    updates = sum(N * N * (c1 - c0) for c0, c1 in zip(s0, s1))
+    value = updates / wall
+    ms_step = wall * 1e3 / args.steps
+
+    ms1, ms2 = b.time_stages(10)
+    cells = N_INST * N * N
+    ach2 = BYTES_STAGE2 * cells / (ms2 * 1e-3) / 1e9
+    ach1 = BYTES_STAGE1 * cells / (ms1 * 1e-3) / 1e9
+    log(f"stage1 {ms1:.3f} ms ({ach1:.0f} GB/s), stage2 {ms2:.3f} ms ({ach2:.0f} GB/s), "
+        f"step {ms_step:.3f} ms, {value / 1e9:.2f} G cell-updates/s")
+    b.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(ics[0], args.cpu_seconds)
+    out = {
+        "metric": "FP64 cell-updates/s (HLLC+vanLeer)", "value": value, "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "random-Riemann batch: 64 instances x 512^2, periodic, HLLC + van Leer MUSCL, "
+                               "SSP-RK2, CFL 0.5, per-instance dt, device-resident loop (hipGraph)",
+                   "instances": N_INST, "n": N, "tuning": tuning},
+        "roofline": {"bound": "hbm", "kernel": "k_stage<HLLC, vanLeer, stage 2>", "achieved": ach2,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach2 / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_cell": BYTES_STAGE2, "launch_ms": ms2, "stage1_launch_ms": ms1,
+                     "stage1_achieved": ach1},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

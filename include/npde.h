@@ -1,0 +1,133 @@
+/* npde: MI355X-native 2D Euler finite-volume step (MUSCL + HLLC/Rusanov + SSP-RK2, binary64).
+ *
+ * C-ABI of libnpde.so.  Plain C types only; every call returns an npde_status and never exits or
+ * throws.  One handle drives one instance or a batch of instances of any sizes; device memory
+ * belongs to the library, host arrays to the caller.  A handle is not re-entrant: use one per host
+ * thread.
+ *
+ * State layouts accepted by set/get:
+ *   NPDE_LAYOUT_PLANES     4 x ny x nx binary64: rho, rho*u, rho*v, E over the interior cells
+ *   NPDE_LAYOUT_GHOSTED    (ny+4) x (nx+4) cells of 4 binary64 {rho, rho*u, rho*v, E}, the reference's
+ *                          std::vector<Conserved> with ng = 2 (grid2d.hpp:12-28); ghosts are ignored
+ *                          on input and filled as apply_bcs would (grid2d.hpp:43-66) on output
+ */
+#ifndef NPDE_H
+#define NPDE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    NPDE_OK = 0,
+    NPDE_ERR_INVALID = 1,       /* bad argument (sizes, enum values, save sizes that do not divide) */
+    NPDE_ERR_NONPHYSICAL = 2,   /* rho <= 0 or p <= 0 met when a step was needed (solver2d.hpp:75) */
+    NPDE_ERR_DEVICE = 3,        /* HIP runtime error */
+    NPDE_ERR_NOMEM = 4,         /* device allocation failed */
+    NPDE_ERR_STATE = 5          /* call out of order (e.g. trajectory requested before a run) */
+} npde_status;
+
+typedef enum { NPDE_BC_TRANSMISSIVE = 0, NPDE_BC_PERIODIC = 1 } npde_bc;        /* grid2d.hpp:7 */
+typedef enum { NPDE_FLUX_RUSANOV = 0, NPDE_FLUX_HLLC = 1 } npde_flux;           /* riemann2d.hpp:8 */
+typedef enum {
+    NPDE_LIM_NONE = 0, NPDE_LIM_UNLIMITED = 1, NPDE_LIM_MINMOD = 2, NPDE_LIM_VANLEER = 3
+} npde_limiter;                                                                 /* limiters.hpp:5 */
+typedef enum { NPDE_LAYOUT_PLANES = 0, NPDE_LAYOUT_GHOSTED = 1 } npde_layout;
+
+/* Grid of one instance: replaces make_grid(nx, ny, x_min, x_max, y_min, y_max, bc_x, bc_y)
+ * (grid2d.hpp:30-39). */
+typedef struct {
+    int nx, ny;
+    double x_min, x_max, y_min, y_max;
+    int bc_x, bc_y; /* npde_bc */
+} npde_grid_desc;
+
+/* First non-physical cell, as Solver::report_bad_cell finds it (row-major scan, solver2d.hpp:116-126). */
+typedef struct {
+    int instance, i, j;
+    double rho, p;
+} npde_bad_cell;
+
+/* Kernel launch configuration (chosen by npde_autotune or set by hand). */
+typedef struct {
+    int rows_per_task;    /* band height of one wavefront's task (>= 1) */
+    int waves_per_block;  /* wavefronts per workgroup: 1, 2, 4 (or 8 within the 256-lane bound) */
+    int steps_per_graph;  /* RK2 steps captured per hipGraph launch in npde_run / npde_advance_to */
+    int use_graph;        /* 0: plain stream launches */
+} npde_tuning;
+
+typedef struct npde_solver npde_solver;
+
+/* ---- lifetime: replaces Solver(const Grid&, FluxFn, Limiter, double cfl) (solver2d.hpp:61-64) ---- */
+npde_status npde_create(const npde_grid_desc* grid, int flux, int limiter, double cfl, npde_solver** out);
+/* A batch of n independent instances of (possibly) different sizes, stepped in the same launches,
+ * each with its own dt, time and frame landing. */
+npde_status npde_batch_create(int n, const npde_grid_desc* grids, int flux, int limiter, double cfl,
+                              npde_solver** out);
+npde_status npde_destroy(npde_solver* s);
+int npde_num_instances(const npde_solver* s);
+const char* npde_last_error(const npde_solver* s);
+
+/* ---- state: the caller's std::vector<Conserved> U (solver2d.hpp:67, 86) ---- */
+/* Setting a state also resets that instance's time and step counter to 0. */
+npde_status npde_set_state(npde_solver* s, int instance, const double* U, int layout);
+npde_status npde_get_state(npde_solver* s, int instance, double* U, int layout);
+/* Device-resident variants (4 x ny x nx planes already in device memory, e.g. a torch tensor). */
+npde_status npde_set_state_device(npde_solver* s, int instance, const double* dU);
+npde_status npde_get_state_device(npde_solver* s, int instance, double* dU);
+npde_status npde_get_time(npde_solver* s, int instance, double* t, long long* steps);
+
+/* ---- the hot path: Solver::compute_dt (solver2d.hpp:67-83) and Solver::step (solver2d.hpp:86-103) ---- */
+/* dt of every instance (array of n); on a non-physical state returns NPDE_ERR_NONPHYSICAL and fills bad. */
+npde_status npde_compute_dt(npde_solver* s, double* dt, npde_bad_cell* bad);
+/* One SSP-RK2 step of every instance with the given dt (array of n; a negative dt skips that instance). */
+npde_status npde_step(npde_solver* s, const double* dt);
+
+/* ---- the driver's time loop (main2d.cpp:224-246), device-resident ---- */
+/* Advance every instance until t >= t_target, each landing exactly with dt = min(dt_cfl, t_target - t). */
+npde_status npde_advance_to(npde_solver* s, double t_target, npde_bad_cell* bad);
+/* As npde_advance_to, but every instance also stops after nsteps more steps (fixed-work timing). */
+npde_status npde_advance_steps(npde_solver* s, long long nsteps, double t_target, npde_bad_cell* bad);
+/* Pre-roll to t_start, then nout frames at t_start + (t_end - t_start) k / nout, recording the block
+ * average of the conserved variables at each of the n_sizes sizes (trajectory.hpp:31-51). */
+npde_status npde_run_trajectory(npde_solver* s, double t_start, double t_end, int nout, const int* sizes,
+                                int n_sizes, npde_bad_cell* bad);
+/* Frames of the last npde_run_trajectory: (nout+1) x 4 x (ny/f) x m float32 for size m. */
+npde_status npde_get_trajectory(npde_solver* s, int instance, int m, float* out);
+npde_status npde_get_frame_times(npde_solver* s, int instance, double* t);
+
+/* Block average of the current state (trajectory.hpp:39-48): 4 x (ny/f) x m float32, f = nx/m. */
+npde_status npde_block_average(npde_solver* s, int instance, int m, float* out);
+
+/* ---- learned flux correction (coarse solver) ---- */
+/* Device pointers (4 x ny x nx binary64 each) holding per-face increments added to the face fluxes
+ * of every subsequent stage: corr_x[q][j][i] on the face left of cell (i, j), corr_y[q][j][i] on the
+ * face below it.  Pass NULLs to switch the correction off. */
+npde_status npde_set_flux_correction(npde_solver* s, int instance, const double* corr_x, const double* corr_y);
+
+/* ---- built-in problems of the driver: sod_x, sod_y, lax_liu3, sine2d, kh (main2d.cpp:52-135) ---- */
+/* Fills grid and t_end; if U is non-NULL also the initial state as 4 x ny x nx planes.  Call once
+ * with U = NULL to learn the grid size. */
+npde_status npde_builtin_problem(const char* test, int n, npde_grid_desc* grid, double* t_end, double* U);
+
+/* ---- performance ---- */
+npde_status npde_get_tuning(const npde_solver* s, npde_tuning* t);
+npde_status npde_set_tuning(npde_solver* s, const npde_tuning* t);
+/* Time candidate launch configurations on this handle's own instances (state is saved and restored)
+ * and keep the fastest; results are cached per (sizes, bc, flux, limiter) in cache_path if non-NULL. */
+npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* chosen);
+/* Average device milliseconds of the stage-1 and stage-2 kernel launches over nsteps host-driven
+ * steps of the current state (HIP events on the handle's stream around each launch; the state advances). */
+npde_status npde_time_stages(npde_solver* s, int nsteps, double* ms_stage1, double* ms_stage2);
+/* Milliseconds of device time of the last npde_step / npde_advance_to / npde_run_trajectory. */
+npde_status npde_last_elapsed_ms(const npde_solver* s, double* ms);
+/* Stream the handle launches on (hipStream_t), for callers that order their own work after it. */
+void* npde_stream(npde_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NPDE_H */

@@ -1,0 +1,876 @@
+// Host side of libnpde.so: the C-ABI declared in include/npde.h.
+//
+// Replaces the reference's in-process plugin API (solver2d.hpp:57-103 Solver; grid2d.hpp:30-66
+// make_grid / apply_bcs; main2d.cpp:224-246 run loop; trajectory.hpp:31-51 block average) with
+// opaque handles over device-resident state.  No call exits or throws: errors come back as
+// npde_status with a message in npde_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/npde.h"
+#include "euler_numerics.hpp"
+#include "npde_device.hpp"
+
+namespace npde {
+hipError_t launch_stage(int scheme, int lim, int stage, bool corr, int waves_per_block, const InstDesc* desc,
+                        Ctl* ctl, const Task* tasks, int ntasks, Partial* part, int* ndone, hipStream_t s);
+hipError_t launch_wavespeed(int waves_per_block, const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks,
+                            Partial* part, int* ndone, hipStream_t s);
+hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int nftasks, int* ndone,
+                        hipStream_t s);
+hipError_t launch_block_average(const InstDesc* desc, int inst, int m, int nyc, float* out, hipStream_t s);
+hipError_t launch_prep_host_step(Ctl* ctl, int n, const double* dts, hipStream_t s);
+}  // namespace npde
+
+using namespace npde;
+
+struct npde_solver {
+    int n = 0;
+    int flux = kHllc, lim = kLimVanLeer;
+    double cfl = 0.5;
+    std::vector<npde_grid_desc> grids;
+    std::vector<InstDesc> hdesc;
+    std::vector<Ctl> hctl;
+    std::vector<Task> htasks;
+    std::vector<FTask> hftasks;
+    std::vector<size_t> state_off;  // offset of each instance's U (U1 follows after 4 planes)
+    InstDesc* ddesc = nullptr;
+    Ctl* dctl = nullptr;
+    Task* dtasks = nullptr;
+    FTask* dftasks = nullptr;
+    Partial* dpart = nullptr;
+    int* dndone = nullptr;
+    double* ddts = nullptr;
+    double* dstate = nullptr;
+    float* dtraj = nullptr;
+    double* dftimes = nullptr;
+    size_t traj_bytes = 0;
+    std::vector<int> traj_sizes;
+    int traj_nout = -1;
+    bool has_corr = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    npde_tuning tune{64, 4, 32, 1};
+    hipGraphExec_t graph = nullptr;
+    int graph_key_frames = -1;  // nftasks the cached graph was captured with (0: no frame kernel)
+    int graph_key_steps = 0;
+    int graph_key_corr = -1;
+    int graph_key_rows = -1, graph_key_wpb = -1;
+    std::string err;
+    double last_ms = 0.0;
+};
+
+namespace {
+
+npde_status fail(npde_solver* s, npde_status st, const std::string& msg) {
+    if (s) s->err = msg;
+    return st;
+}
+
+#define HIPCHK(s, call)                                                                              \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return fail((s), NPDE_ERR_DEVICE, std::string(#call ": ") + hipGetErrorString(e_));     \
+    } while (0)
+
+size_t plane_stride(const npde_grid_desc& g) {
+    const size_t cells = size_t(g.nx) * g.ny;
+    return (cells + 63) / 64 * 64;  // 512-byte aligned planes
+}
+
+bool valid_grid(const npde_grid_desc& g, std::string& why) {
+    if (g.nx <= 0 || g.ny <= 0) { why = "nx and ny must be positive"; return false; }
+    if (double(g.nx) * g.ny >= 2147483647.0) { why = "nx*ny must fit in int32"; return false; }
+    if (!(g.x_max > g.x_min) || !(g.y_max > g.y_min)) { why = "empty domain"; return false; }
+    if ((g.bc_x != 0 && g.bc_x != 1) || (g.bc_y != 0 && g.bc_y != 1)) { why = "bc must be 0 or 1"; return false; }
+    return true;
+}
+
+void build_tasks(npde_solver* s) {
+    s->htasks.clear();
+    const int rows = std::max(1, s->tune.rows_per_task);
+    for (int k = 0; k < s->n; ++k) {
+        InstDesc& D = s->hdesc[k];
+        D.task0 = int(s->htasks.size());
+        const int strips = (D.nx + kStripCols - 1) / kStripCols;
+        for (int j0 = 0; j0 < D.ny; j0 += rows)
+            for (int st = 0; st < strips; ++st)
+                s->htasks.push_back({k, st * kStripCols, j0, std::min(j0 + rows, D.ny)});
+        D.ntasks = int(s->htasks.size()) - D.task0;
+    }
+}
+
+npde_status upload_tasks(npde_solver* s) {
+    build_tasks(s);
+    if (s->dtasks) (void)hipFree(s->dtasks);
+    if (s->dpart) (void)hipFree(s->dpart);
+    s->dtasks = nullptr;
+    s->dpart = nullptr;
+    HIPCHK(s, hipMalloc(&s->dtasks, sizeof(Task) * s->htasks.size()));
+    HIPCHK(s, hipMalloc(&s->dpart, sizeof(Partial) * s->htasks.size()));
+    HIPCHK(s, hipMemcpy(s->dtasks, s->htasks.data(), sizeof(Task) * s->htasks.size(), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->ddesc, s->hdesc.data(), sizeof(InstDesc) * s->n, hipMemcpyHostToDevice));
+    return NPDE_OK;
+}
+
+npde_status pull_ctl(npde_solver* s) {
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(s->hctl.data(), s->dctl, sizeof(Ctl) * s->n, hipMemcpyDeviceToHost));
+    return NPDE_OK;
+}
+
+npde_status push_ctl(npde_solver* s) {
+    HIPCHK(s, hipMemcpy(s->dctl, s->hctl.data(), sizeof(Ctl) * s->n, hipMemcpyHostToDevice));
+    return NPDE_OK;
+}
+
+void drop_graph(npde_solver* s) {
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    s->graph_key_frames = -1;
+}
+
+// to_primitive on the host (same operations as the device), for error messages
+Prim host_prim(const double* c) { return prim_of(c[0], c[1], c[2], c[3]); }
+
+npde_status fill_bad(npde_solver* s, npde_bad_cell* bad) {
+    for (int k = 0; k < s->n; ++k) {
+        const Ctl& c = s->hctl[k];
+        if (c.error || (c.mode == kModeHost && c.bad != kNoBad)) {
+            const InstDesc& D = s->hdesc[k];
+            const int i = c.bad % D.nx, j = c.bad / D.nx;
+            double cell[4];
+            for (int q = 0; q < 4; ++q)
+                (void)hipMemcpy(&cell[q], D.U + size_t(q) * D.plane + c.bad, sizeof(double), hipMemcpyDeviceToHost);
+            const Prim w = host_prim(cell);
+            if (bad) *bad = {k, i, j, w.d, w.p};
+            char buf[256];
+            std::snprintf(buf, sizeof buf, "Non-physical state in instance %d, cell (%d, %d): rho = %g, p = %g", k,
+                          i, j, w.d, w.p);
+            return fail(s, NPDE_ERR_NONPHYSICAL, buf);
+        }
+    }
+    return NPDE_OK;
+}
+
+npde_status launch_step_pair(npde_solver* s) {
+    HIPCHK(s, launch_stage(s->flux, s->lim, 1, s->has_corr, s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks,
+                           int(s->htasks.size()), s->dpart, s->dndone, s->stream));
+    HIPCHK(s, launch_stage(s->flux, s->lim, 2, s->has_corr, s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks,
+                           int(s->htasks.size()), s->dpart, s->dndone, s->stream));
+    return NPDE_OK;
+}
+
+npde_status launch_iterations(npde_solver* s, int iters, int nft) {
+    for (int it = 0; it < iters; ++it) {
+        if (nft > 0) HIPCHK(s, launch_frame(s->ddesc, s->dctl, s->dftasks, nft, s->dndone, s->stream));
+        npde_status st = launch_step_pair(s);
+        if (st != NPDE_OK) return st;
+    }
+    return NPDE_OK;
+}
+
+npde_status ensure_graph(npde_solver* s, int nft) {
+    const int K = std::max(1, s->tune.steps_per_graph);
+    if (s->graph && s->graph_key_frames == nft && s->graph_key_steps == K && s->graph_key_corr == int(s->has_corr) &&
+        s->graph_key_rows == s->tune.rows_per_task && s->graph_key_wpb == s->tune.waves_per_block)
+        return NPDE_OK;
+    drop_graph(s);
+    hipGraph_t g = nullptr;
+    HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    npde_status st = launch_iterations(s, K, nft);
+    hipError_t e = hipStreamEndCapture(s->stream, &g);
+    if (st != NPDE_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return fail(s, NPDE_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        s->graph = nullptr;
+        return fail(s, NPDE_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    s->graph_key_frames = nft;
+    s->graph_key_steps = K;
+    s->graph_key_corr = int(s->has_corr);
+    s->graph_key_rows = s->tune.rows_per_task;
+    s->graph_key_wpb = s->tune.waves_per_block;
+    return NPDE_OK;
+}
+
+// Device-resident loop shared by npde_advance_to and npde_run_trajectory.
+npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, bool record, npde_bad_cell* bad,
+                     long long max_new_steps = -1) {
+    npde_status st = pull_ctl(s);
+    if (st != NPDE_OK) return st;
+    for (auto& c : s->hctl) {
+        c.step_limit = max_new_steps < 0 ? (long long)(~0ull >> 1) : c.steps + max_new_steps;
+        c.t_start = t_start;
+        c.t_end = t_end;
+        c.nout = nout;
+        c.k = 0;
+        c.mode = kModeAuto;
+        c.record = record ? 1 : 0;
+        c.pending = -1;
+        c.done = 0;
+        c.error = 0;
+        c.active = 0;
+        c.count = 0;
+        c.fcount = 0;
+    }
+    if ((st = push_ctl(s)) != NPDE_OK) return st;
+    HIPCHK(s, hipMemsetAsync(s->dndone, 0, sizeof(int), s->stream));
+    const int nft = record ? int(s->hftasks.size()) : 0;
+    if (s->tune.use_graph) {
+        if ((st = ensure_graph(s, nft)) != NPDE_OK) return st;
+    }
+    HIPCHK(s, hipEventRecord(s->ev0, s->stream));
+    HIPCHK(s, launch_wavespeed(s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks, int(s->htasks.size()),
+                               s->dpart, s->dndone, s->stream));
+    int done = 0;
+    const int K = std::max(1, s->tune.steps_per_graph);
+    // A state whose wave speed is infinite gives dt = 0 and would never reach its target (the reference
+    // loops forever there too); bound the loop instead.
+    const long long max_chunks = std::max(1LL, 50000000LL / K);
+    for (long long chunk = 0;; ++chunk) {
+        if (chunk >= max_chunks) return fail(s, NPDE_ERR_STATE, "run did not finish within 5e7 steps");
+        if (s->tune.use_graph) {
+            HIPCHK(s, hipGraphLaunch(s->graph, s->stream));
+        } else if ((st = launch_iterations(s, K, nft)) != NPDE_OK) {
+            return st;
+        }
+        HIPCHK(s, hipMemcpyAsync(&done, s->dndone, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        if (done >= s->n) break;
+    }
+    HIPCHK(s, hipEventRecord(s->ev1, s->stream));
+    HIPCHK(s, hipEventSynchronize(s->ev1));
+    float ms = 0.f;
+    HIPCHK(s, hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->last_ms = ms;
+    if ((st = pull_ctl(s)) != NPDE_OK) return st;
+    return fill_bad(s, bad);
+}
+
+npde_status set_host_mode(npde_solver* s) {
+    npde_status st = pull_ctl(s);
+    if (st != NPDE_OK) return st;
+    for (auto& c : s->hctl) {
+        c.mode = kModeHost;
+        c.done = 0;
+        c.error = 0;
+        c.active = 0;
+        c.pending = -1;
+        c.count = 0;
+        c.fcount = 0;
+    }
+    return push_ctl(s);
+}
+
+void fill_ghosts(const npde_grid_desc& g, double* A) {
+    // A: (ny+4) x (nx+4) x 4, physical cells already set; x ghosts on physical rows, then y ghosts
+    // over all columns (so the corners are filled too), as grid2d.hpp:43-66 does.
+    const int ng = 2, W = g.nx + 2 * ng;
+    auto at = [&](int i, int j) { return A + (size_t(j + ng) * W + (i + ng)) * 4; };
+    for (int j = 0; j < g.ny; ++j)
+        for (int k = 1; k <= ng; ++k) {
+            const int il = g.bc_x == kPeriodic ? g.nx - k : 0;
+            const int ir = g.bc_x == kPeriodic ? k - 1 : g.nx - 1;
+            std::memcpy(at(-k, j), at(il, j), 32);
+            std::memcpy(at(g.nx - 1 + k, j), at(ir, j), 32);
+        }
+    for (int i = -ng; i < g.nx + ng; ++i)
+        for (int k = 1; k <= ng; ++k) {
+            const int jb = g.bc_y == kPeriodic ? g.ny - k : 0;
+            const int jt = g.bc_y == kPeriodic ? k - 1 : g.ny - 1;
+            std::memcpy(at(i, -k), at(i, jb), 32);
+            std::memcpy(at(i, g.ny - 1 + k), at(i, jt), 32);
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+npde_status npde_batch_create(int n, const npde_grid_desc* grids, int flux, int limiter, double cfl,
+                              npde_solver** out) {
+    if (!out || !grids || n <= 0) return NPDE_ERR_INVALID;
+    *out = nullptr;
+    if (flux != kRusanov && flux != kHllc) return NPDE_ERR_INVALID;
+    if (limiter < kLimNone || limiter > kLimVanLeer) return NPDE_ERR_INVALID;
+    if (!(cfl > 0.0)) return NPDE_ERR_INVALID;
+    std::string why;
+    for (int k = 0; k < n; ++k)
+        if (!valid_grid(grids[k], why)) return NPDE_ERR_INVALID;
+    auto* s = new npde_solver;
+    s->n = n;
+    s->flux = flux;
+    s->lim = limiter;
+    s->cfl = cfl;
+    s->grids.assign(grids, grids + n);
+    s->hdesc.resize(n);
+    s->hctl.assign(n, Ctl{});
+    s->state_off.resize(n);
+    size_t total = 0;
+    for (int k = 0; k < n; ++k) {
+        s->state_off[k] = total;
+        total += 8 * plane_stride(grids[k]);
+    }
+    auto cleanup = [&](npde_status st) {
+        npde_destroy(s);
+        return st;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(NPDE_ERR_DEVICE);
+    if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
+        return cleanup(NPDE_ERR_DEVICE);
+    if (hipMalloc(&s->dstate, total * sizeof(double)) != hipSuccess) return cleanup(NPDE_ERR_NOMEM);
+    if (hipMemset(s->dstate, 0, total * sizeof(double)) != hipSuccess) return cleanup(NPDE_ERR_DEVICE);
+    if (hipMalloc(&s->ddesc, sizeof(InstDesc) * n) != hipSuccess || hipMalloc(&s->dctl, sizeof(Ctl) * n) != hipSuccess ||
+        hipMalloc(&s->dndone, sizeof(int)) != hipSuccess || hipMalloc(&s->ddts, sizeof(double) * n) != hipSuccess)
+        return cleanup(NPDE_ERR_NOMEM);
+    for (int k = 0; k < n; ++k) {
+        const npde_grid_desc& g = grids[k];
+        InstDesc& D = s->hdesc[k];
+        std::memset(&D, 0, sizeof D);
+        const size_t ps = plane_stride(g);
+        D.U = s->dstate + s->state_off[k];
+        D.U1 = D.U + 4 * ps;
+        D.plane = (long long)ps;
+        D.nx = g.nx;
+        D.ny = g.ny;
+        D.bc_x = g.bc_x;
+        D.bc_y = g.bc_y;
+        D.dx = (g.x_max - g.x_min) / g.nx;  // make_grid (grid2d.hpp:37-38)
+        D.dy = (g.y_max - g.y_min) / g.ny;
+        D.cx = 1.0 / D.dx;
+        D.cy = 1.0 / D.dy;
+        D.cfl = cfl;
+        Ctl& c = s->hctl[k];
+        std::memset(&c, 0, sizeof c);
+        c.bad = kNoBad;
+        c.pending = -1;
+        c.mode = kModeHost;
+    }
+    if (upload_tasks(s) != NPDE_OK) return cleanup(NPDE_ERR_DEVICE);
+    if (push_ctl(s) != NPDE_OK) return cleanup(NPDE_ERR_DEVICE);
+    *out = s;
+    return NPDE_OK;
+}
+
+npde_status npde_create(const npde_grid_desc* grid, int flux, int limiter, double cfl, npde_solver** out) {
+    return npde_batch_create(1, grid, flux, limiter, cfl, out);
+}
+
+npde_status npde_destroy(npde_solver* s) {
+    if (!s) return NPDE_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    drop_graph(s);
+    for (void* p : {(void*)s->ddesc, (void*)s->dctl, (void*)s->dtasks, (void*)s->dftasks, (void*)s->dpart,
+                    (void*)s->dndone, (void*)s->ddts, (void*)s->dstate, (void*)s->dtraj, (void*)s->dftimes})
+        if (p) (void)hipFree(p);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return NPDE_OK;
+}
+
+int npde_num_instances(const npde_solver* s) { return s ? s->n : 0; }
+
+const char* npde_last_error(const npde_solver* s) { return s ? s->err.c_str() : "null handle"; }
+
+npde_status npde_set_state(npde_solver* s, int inst, const double* U, int layout) {
+    if (!s || !U || inst < 0 || inst >= s->n) return fail(s, NPDE_ERR_INVALID, "bad instance or null state");
+    const npde_grid_desc& g = s->grids[inst];
+    const size_t cells = size_t(g.nx) * g.ny;
+    const InstDesc& D = s->hdesc[inst];
+    npde_status st = pull_ctl(s);
+    if (st != NPDE_OK) return st;
+    if (layout == NPDE_LAYOUT_PLANES) {
+        for (int q = 0; q < 4; ++q)
+            HIPCHK(s, hipMemcpy(D.U + q * D.plane, U + q * cells, cells * sizeof(double), hipMemcpyHostToDevice));
+    } else if (layout == NPDE_LAYOUT_GHOSTED) {
+        std::vector<double> tmp(4 * cells);
+        const int W = g.nx + 4;
+        for (int j = 0; j < g.ny; ++j)
+            for (int i = 0; i < g.nx; ++i)
+                for (int q = 0; q < 4; ++q)
+                    tmp[q * cells + size_t(j) * g.nx + i] = U[(size_t(j + 2) * W + (i + 2)) * 4 + q];
+        for (int q = 0; q < 4; ++q)
+            HIPCHK(s, hipMemcpy(D.U + q * D.plane, tmp.data() + q * cells, cells * sizeof(double),
+                                hipMemcpyHostToDevice));
+    } else {
+        return fail(s, NPDE_ERR_INVALID, "unknown layout");
+    }
+    Ctl& c = s->hctl[inst];
+    c.t = 0.0;
+    c.steps = 0;
+    c.active = 0;
+    c.bad = kNoBad;
+    c.error = 0;
+    c.done = 0;
+    return push_ctl(s);
+}
+
+npde_status npde_get_state(npde_solver* s, int inst, double* U, int layout) {
+    if (!s || !U || inst < 0 || inst >= s->n) return fail(s, NPDE_ERR_INVALID, "bad instance or null state");
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    const npde_grid_desc& g = s->grids[inst];
+    const size_t cells = size_t(g.nx) * g.ny;
+    const InstDesc& D = s->hdesc[inst];
+    if (layout == NPDE_LAYOUT_PLANES) {
+        for (int q = 0; q < 4; ++q)
+            HIPCHK(s, hipMemcpy(U + q * cells, D.U + q * D.plane, cells * sizeof(double), hipMemcpyDeviceToHost));
+        return NPDE_OK;
+    }
+    if (layout != NPDE_LAYOUT_GHOSTED) return fail(s, NPDE_ERR_INVALID, "unknown layout");
+    std::vector<double> tmp(4 * cells);
+    for (int q = 0; q < 4; ++q)
+        HIPCHK(s, hipMemcpy(tmp.data() + q * cells, D.U + q * D.plane, cells * sizeof(double), hipMemcpyDeviceToHost));
+    const int W = g.nx + 4;
+    std::fill(U, U + size_t(W) * (g.ny + 4) * 4, 0.0);
+    for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i)
+            for (int q = 0; q < 4; ++q) U[(size_t(j + 2) * W + (i + 2)) * 4 + q] = tmp[q * cells + size_t(j) * g.nx + i];
+    fill_ghosts(g, U);
+    return NPDE_OK;
+}
+
+npde_status npde_set_state_device(npde_solver* s, int inst, const double* dU) {
+    if (!s || !dU || inst < 0 || inst >= s->n) return fail(s, NPDE_ERR_INVALID, "bad instance or null state");
+    const npde_grid_desc& g = s->grids[inst];
+    const size_t cells = size_t(g.nx) * g.ny;
+    const InstDesc& D = s->hdesc[inst];
+    npde_status st = pull_ctl(s);
+    if (st != NPDE_OK) return st;
+    HIPCHK(s, hipMemcpy2DAsync(D.U, D.plane * sizeof(double), dU, cells * sizeof(double), cells * sizeof(double), 4,
+                               hipMemcpyDeviceToDevice, s->stream));
+    Ctl& c = s->hctl[inst];
+    c.t = 0.0;
+    c.steps = 0;
+    c.active = 0;
+    c.bad = kNoBad;
+    c.error = 0;
+    c.done = 0;
+    return push_ctl(s);
+}
+
+npde_status npde_get_state_device(npde_solver* s, int inst, double* dU) {
+    if (!s || !dU || inst < 0 || inst >= s->n) return fail(s, NPDE_ERR_INVALID, "bad instance or null state");
+    const npde_grid_desc& g = s->grids[inst];
+    const size_t cells = size_t(g.nx) * g.ny;
+    const InstDesc& D = s->hdesc[inst];
+    HIPCHK(s, hipMemcpy2DAsync(dU, cells * sizeof(double), D.U, D.plane * sizeof(double), cells * sizeof(double), 4,
+                               hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return NPDE_OK;
+}
+
+npde_status npde_get_time(npde_solver* s, int inst, double* t, long long* steps) {
+    if (!s || inst < 0 || inst >= s->n) return fail(s, NPDE_ERR_INVALID, "bad instance");
+    npde_status st = pull_ctl(s);
+    if (st != NPDE_OK) return st;
+    if (t) *t = s->hctl[inst].t;
+    if (steps) *steps = s->hctl[inst].steps;
+    return NPDE_OK;
+}
+
+npde_status npde_compute_dt(npde_solver* s, double* dt, npde_bad_cell* bad) {
+    if (!s || !dt) return fail(s, NPDE_ERR_INVALID, "null argument");
+    npde_status st = set_host_mode(s);
+    if (st != NPDE_OK) return st;
+    HIPCHK(s, launch_wavespeed(s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks, int(s->htasks.size()),
+                               s->dpart, s->dndone, s->stream));
+    if ((st = pull_ctl(s)) != NPDE_OK) return st;
+    for (int k = 0; k < s->n; ++k) {
+        const Ctl& c = s->hctl[k];
+        const InstDesc& D = s->hdesc[k];
+        dt[k] = D.cfl / (c.sx / D.dx + c.sy / D.dy);  // solver2d.hpp:82
+    }
+    return fill_bad(s, bad);
+}
+
+npde_status npde_step(npde_solver* s, const double* dt) {
+    if (!s || !dt) return fail(s, NPDE_ERR_INVALID, "null argument");
+    npde_status st = set_host_mode(s);
+    if (st != NPDE_OK) return st;
+    HIPCHK(s, hipMemcpyAsync(s->ddts, dt, sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(s, launch_prep_host_step(s->dctl, s->n, s->ddts, s->stream));
+    HIPCHK(s, hipEventRecord(s->ev0, s->stream));
+    if ((st = launch_step_pair(s)) != NPDE_OK) return st;
+    HIPCHK(s, hipEventRecord(s->ev1, s->stream));
+    HIPCHK(s, hipEventSynchronize(s->ev1));
+    float ms = 0.f;
+    HIPCHK(s, hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->last_ms = ms;
+    return NPDE_OK;
+}
+
+npde_status npde_advance_to(npde_solver* s, double t_target, npde_bad_cell* bad) {
+    if (!s) return NPDE_ERR_INVALID;
+    return auto_run(s, t_target, t_target, 0, false, bad);
+}
+
+npde_status npde_advance_steps(npde_solver* s, long long nsteps, double t_target, npde_bad_cell* bad) {
+    if (!s || nsteps < 0) return fail(s, NPDE_ERR_INVALID, "bad step count");
+    return auto_run(s, t_target, t_target, 0, false, bad, nsteps);
+}
+
+npde_status npde_time_stages(npde_solver* s, int nsteps, double* ms_stage1, double* ms_stage2) {
+    if (!s || nsteps <= 0 || !ms_stage1 || !ms_stage2) return fail(s, NPDE_ERR_INVALID, "bad arguments");
+    std::vector<double> dts(s->n);
+    npde_status st = npde_compute_dt(s, dts.data(), nullptr);
+    if (st != NPDE_OK) return st;
+    if ((st = set_host_mode(s)) != NPDE_OK) return st;
+    HIPCHK(s, hipMemcpyAsync(s->ddts, dts.data(), sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
+    hipEvent_t e[3];
+    for (auto& x : e) HIPCHK(s, hipEventCreate(&x));
+    double t1 = 0.0, t2 = 0.0;
+    for (int k = 0; k < nsteps && st == NPDE_OK; ++k) {
+        HIPCHK(s, launch_prep_host_step(s->dctl, s->n, s->ddts, s->stream));
+        HIPCHK(s, hipEventRecord(e[0], s->stream));
+        HIPCHK(s, launch_stage(s->flux, s->lim, 1, s->has_corr, s->tune.waves_per_block, s->ddesc, s->dctl,
+                               s->dtasks, int(s->htasks.size()), s->dpart, s->dndone, s->stream));
+        HIPCHK(s, hipEventRecord(e[1], s->stream));
+        HIPCHK(s, launch_stage(s->flux, s->lim, 2, s->has_corr, s->tune.waves_per_block, s->ddesc, s->dctl,
+                               s->dtasks, int(s->htasks.size()), s->dpart, s->dndone, s->stream));
+        HIPCHK(s, hipEventRecord(e[2], s->stream));
+        HIPCHK(s, hipEventSynchronize(e[2]));
+        float a = 0.f, b = 0.f;
+        HIPCHK(s, hipEventElapsedTime(&a, e[0], e[1]));
+        HIPCHK(s, hipEventElapsedTime(&b, e[1], e[2]));
+        t1 += a;
+        t2 += b;
+    }
+    for (auto& x : e) (void)hipEventDestroy(x);
+    *ms_stage1 = t1 / nsteps;
+    *ms_stage2 = t2 / nsteps;
+    return st;
+}
+
+npde_status npde_run_trajectory(npde_solver* s, double t_start, double t_end, int nout, const int* sizes, int n_sizes,
+                                npde_bad_cell* bad) {
+    if (!s || nout <= 0 || n_sizes < 0 || n_sizes > kMaxLevels || (n_sizes > 0 && !sizes))
+        return fail(s, NPDE_ERR_INVALID, "bad run arguments");
+    if (!(t_start >= 0.0 && t_start < t_end)) return fail(s, NPDE_ERR_INVALID, "need 0 <= t_start < t_end");
+    // trajectory levels: m must divide nx, nx/m must divide ny (trajectory.hpp:22-29)
+    size_t total = 0;
+    std::vector<std::vector<size_t>> off(s->n, std::vector<size_t>(kMaxLevels, 0));
+    for (int k = 0; k < s->n; ++k) {
+        const npde_grid_desc& g = s->grids[k];
+        for (int l = 0; l < n_sizes; ++l) {
+            const int m = sizes[l];
+            if (m <= 0 || g.nx % m != 0)
+                return fail(s, NPDE_ERR_INVALID, "save size " + std::to_string(m) + " does not divide nx = " +
+                                                     std::to_string(g.nx));
+            const int f = g.nx / m;
+            if (g.ny % f != 0)
+                return fail(s, NPDE_ERR_INVALID, "save size " + std::to_string(m) + " does not divide ny evenly");
+            off[k][l] = total;
+            total += size_t(nout + 1) * 4 * m * (g.ny / f);
+        }
+    }
+    if (s->dtraj) (void)hipFree(s->dtraj);
+    s->dtraj = nullptr;
+    s->traj_bytes = total * sizeof(float);
+    if (total) {
+        if (hipMalloc(&s->dtraj, s->traj_bytes) != hipSuccess) return fail(s, NPDE_ERR_NOMEM, "trajectory buffer");
+        HIPCHK(s, hipMemset(s->dtraj, 0, s->traj_bytes));
+    }
+    s->traj_sizes.assign(sizes, sizes + n_sizes);
+    s->traj_nout = nout;
+    if (s->dftimes) (void)hipFree(s->dftimes);
+    s->dftimes = nullptr;
+    if (hipMalloc(&s->dftimes, sizeof(double) * size_t(nout + 1) * s->n) != hipSuccess)
+        return fail(s, NPDE_ERR_NOMEM, "frame times");
+    HIPCHK(s, hipMemset(s->dftimes, 0, sizeof(double) * size_t(nout + 1) * s->n));
+    s->hftasks.clear();
+    for (int k = 0; k < s->n; ++k) {
+        InstDesc& D = s->hdesc[k];
+        D.nlevels = n_sizes;
+        D.ftimes = s->dftimes + size_t(k) * (nout + 1);
+        D.ftask0 = int(s->hftasks.size());
+        for (int l = 0; l < n_sizes; ++l) {
+            const int m = sizes[l], f = D.nx / m, cells = m * (D.ny / f);
+            D.lvl_m[l] = m;
+            D.lvl_f[l] = f;
+            D.traj[l] = s->dtraj + off[k][l];
+            for (int c0 = 0; c0 < cells; c0 += 256) s->hftasks.push_back({k, l, c0, 0});
+        }
+        D.nftasks = int(s->hftasks.size()) - D.ftask0;
+    }
+    // an instance with no levels still needs one (empty) frame task so its frames complete
+    for (int k = 0; k < s->n && n_sizes == 0; ++k) {
+        InstDesc& D = s->hdesc[k];
+        D.ftask0 = int(s->hftasks.size());
+        s->hftasks.push_back({k, 0, 1 << 30, 0});
+        D.nftasks = 1;
+        D.lvl_m[0] = 1;
+        D.lvl_f[0] = D.nx;
+    }
+    if (s->dftasks) (void)hipFree(s->dftasks);
+    s->dftasks = nullptr;
+    HIPCHK(s, hipMalloc(&s->dftasks, sizeof(FTask) * s->hftasks.size()));
+    HIPCHK(s, hipMemcpy(s->dftasks, s->hftasks.data(), sizeof(FTask) * s->hftasks.size(), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->ddesc, s->hdesc.data(), sizeof(InstDesc) * s->n, hipMemcpyHostToDevice));
+    drop_graph(s);
+    return auto_run(s, t_start, t_end, nout, true, bad);
+}
+
+npde_status npde_get_trajectory(npde_solver* s, int inst, int m, float* out) {
+    if (!s || !out || inst < 0 || inst >= s->n) return fail(s, NPDE_ERR_INVALID, "bad arguments");
+    if (!s->dtraj) return fail(s, NPDE_ERR_STATE, "no trajectory recorded");
+    const InstDesc& D = s->hdesc[inst];
+    for (int l = 0; l < D.nlevels; ++l)
+        if (D.lvl_m[l] == m) {
+            const size_t n = size_t(s->traj_nout + 1) * 4 * m * (D.ny / D.lvl_f[l]);
+            HIPCHK(s, hipMemcpy(out, D.traj[l], n * sizeof(float), hipMemcpyDeviceToHost));
+            return NPDE_OK;
+        }
+    return fail(s, NPDE_ERR_INVALID, "size not recorded");
+}
+
+npde_status npde_get_frame_times(npde_solver* s, int inst, double* t) {
+    if (!s || !t || inst < 0 || inst >= s->n) return fail(s, NPDE_ERR_INVALID, "bad arguments");
+    if (s->traj_nout < 0 || !s->dftimes) return fail(s, NPDE_ERR_STATE, "no trajectory recorded");
+    HIPCHK(s, hipMemcpy(t, s->hdesc[inst].ftimes, sizeof(double) * (s->traj_nout + 1), hipMemcpyDeviceToHost));
+    return NPDE_OK;
+}
+
+npde_status npde_block_average(npde_solver* s, int inst, int m, float* out) {
+    if (!s || !out || inst < 0 || inst >= s->n) return fail(s, NPDE_ERR_INVALID, "bad arguments");
+    const npde_grid_desc& g = s->grids[inst];
+    if (m <= 0 || g.nx % m != 0 || g.ny % (g.nx / m) != 0) return fail(s, NPDE_ERR_INVALID, "size does not divide");
+    const int nyc = g.ny / (g.nx / m);
+    float* d = nullptr;
+    const size_t n = size_t(4) * m * nyc;
+    if (hipMalloc(&d, n * sizeof(float)) != hipSuccess) return fail(s, NPDE_ERR_NOMEM, "block average buffer");
+    hipError_t e = launch_block_average(s->ddesc, inst, m, nyc, d, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, n * sizeof(float), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(s, NPDE_ERR_DEVICE, hipGetErrorString(e));
+    return NPDE_OK;
+}
+
+npde_status npde_set_flux_correction(npde_solver* s, int inst, const double* corr_x, const double* corr_y) {
+    if (!s || inst < 0 || inst >= s->n) return fail(s, NPDE_ERR_INVALID, "bad instance");
+    if ((corr_x == nullptr) != (corr_y == nullptr)) return fail(s, NPDE_ERR_INVALID, "give both corrections or none");
+    InstDesc& D = s->hdesc[inst];
+    D.corr_x = corr_x;
+    D.corr_y = corr_y;
+    bool any = false;
+    for (const auto& d : s->hdesc) any = any || d.corr_x != nullptr;
+    if (any) {
+        for (const auto& d : s->hdesc)
+            if (!d.corr_x) return fail(s, NPDE_ERR_STATE, "set a correction on every instance of the batch (or none)");
+        for (const auto& d : s->hdesc)
+            if (d.plane != (long long)(size_t(d.nx) * d.ny))
+                return fail(s, NPDE_ERR_INVALID, "flux correction needs nx*ny to be a multiple of 64");
+    }
+    s->has_corr = any;
+    HIPCHK(s, hipMemcpy(s->ddesc, s->hdesc.data(), sizeof(InstDesc) * s->n, hipMemcpyHostToDevice));
+    drop_graph(s);
+    return NPDE_OK;
+}
+
+npde_status npde_get_tuning(const npde_solver* s, npde_tuning* t) {
+    if (!s || !t) return NPDE_ERR_INVALID;
+    *t = s->tune;
+    return NPDE_OK;
+}
+
+npde_status npde_set_tuning(npde_solver* s, const npde_tuning* t) {
+    if (!s || !t) return NPDE_ERR_INVALID;
+    if (t->rows_per_task < 1 || !(t->waves_per_block == 1 || t->waves_per_block == 2 || t->waves_per_block == 4) ||
+        t->steps_per_graph < 1)
+        return fail(s, NPDE_ERR_INVALID, "bad tuning");
+    const bool rows_changed = t->rows_per_task != s->tune.rows_per_task;
+    s->tune = *t;
+    drop_graph(s);
+    if (rows_changed) return upload_tasks(s);
+    return NPDE_OK;
+}
+
+npde_status npde_last_elapsed_ms(const npde_solver* s, double* ms) {
+    if (!s || !ms) return NPDE_ERR_INVALID;
+    *ms = s->last_ms;
+    return NPDE_OK;
+}
+
+void* npde_stream(npde_solver* s) { return s ? (void*)s->stream : nullptr; }
+
+// Built-in initial conditions of the driver (main2d.cpp:52-135), evaluated with the host libm.
+npde_status npde_builtin_problem(const char* test, int n, npde_grid_desc* grid, double* t_end, double* U) {
+    if (!test || !grid || !t_end || n <= 0) return NPDE_ERR_INVALID;
+    const std::string name(test);
+    const double PI = 3.14159265358979323846;
+    npde_grid_desc g{};
+    if (name == "sod_x" || name == "sod_y") {
+        const int m = 4;
+        g = name == "sod_x" ? npde_grid_desc{n, m, 0.0, 1.0, 0.0, double(m) / n, kTransmissive, kTransmissive}
+                            : npde_grid_desc{m, n, 0.0, double(m) / n, 0.0, 1.0, kTransmissive, kTransmissive};
+        *t_end = 0.2;
+    } else if (name == "lax_liu3") {
+        g = {n, n, 0.0, 1.0, 0.0, 1.0, kTransmissive, kTransmissive};
+        *t_end = 0.3;
+    } else if (name == "sine2d") {
+        g = {n, n, 0.0, 1.0, 0.0, 1.0, kPeriodic, kPeriodic};
+        *t_end = 1.0;
+    } else if (name == "kh") {
+        g = {n, n, 0.0, 1.0, 0.0, 1.0, kPeriodic, kPeriodic};
+        *t_end = 2.0;
+    } else {
+        return NPDE_ERR_INVALID;
+    }
+    *grid = g;
+    if (!U) return NPDE_OK;
+    const double dx = (g.x_max - g.x_min) / g.nx, dy = (g.y_max - g.y_min) / g.ny;
+    const size_t cells = size_t(g.nx) * g.ny;
+    for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i) {
+            const double x = g.x_min + (i + 0.5) * dx, y = g.y_min + (j + 0.5) * dy;  // Grid::xc / yc
+            Prim w{};
+            if (name == "sod_x" || name == "sod_y") {
+                const double pos = name == "sod_x" ? x : y;
+                w = pos < 0.5 ? Prim{1.0, 0.0, 0.0, 1.0} : Prim{0.125, 0.0, 0.0, 0.1};
+            } else if (name == "lax_liu3") {  // Lax & Liu configuration 3
+                const bool right = x > 0.5, top = y > 0.5;
+                w = top ? (right ? Prim{1.5, 0.0, 0.0, 1.5} : Prim{0.5323, 1.206, 0.0, 0.3})
+                        : (right ? Prim{0.5323, 0.0, 1.206, 0.3} : Prim{0.138, 1.206, 1.206, 0.029});
+            } else if (name == "sine2d") {
+                w = {1.0 + 0.2 * std::sin(2.0 * PI * (x + y)), 1.0, 1.0, 1.0};
+            } else {  // kh: two exponential shear layers (McNally, Lyra & Passy 2012)
+                const double r1 = 1.0, r2 = 2.0, rm = 0.5 * (r1 - r2);
+                const double u1 = 0.5, u2 = -0.5, um = 0.5 * (u1 - u2);
+                const double L = 0.025;
+                double rho, u;
+                if (y < 0.25) {
+                    rho = r1 - rm * std::exp((y - 0.25) / L);
+                    u = u1 - um * std::exp((y - 0.25) / L);
+                } else if (y < 0.5) {
+                    rho = r2 + rm * std::exp((0.25 - y) / L);
+                    u = u2 + um * std::exp((0.25 - y) / L);
+                } else if (y < 0.75) {
+                    rho = r2 + rm * std::exp((y - 0.75) / L);
+                    u = u2 + um * std::exp((y - 0.75) / L);
+                } else {
+                    rho = r1 - rm * std::exp((0.75 - y) / L);
+                    u = u1 - um * std::exp((0.75 - y) / L);
+                }
+                w = {rho, u, 0.01 * std::sin(4.0 * PI * x), 2.5};
+            }
+            const Cons c = cons_of(w);
+            const size_t k = size_t(j) * g.nx + i;
+            U[k] = c.d;
+            U[cells + k] = c.m;
+            U[2 * cells + k] = c.n;
+            U[3 * cells + k] = c.e;
+        }
+    return NPDE_OK;
+}
+
+// Autotuner: times candidate (rows_per_task, waves_per_block) pairs on the handle's own instances
+// with host-driven steps of the current CFL dt, restoring the state after every trial.  Every
+// candidate computes the same bits (the per-cell arithmetic does not depend on the launch shape);
+// only the time differs.
+npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* chosen) {
+    if (!s) return NPDE_ERR_INVALID;
+    std::ostringstream key;
+    key << "flux" << s->flux << "_lim" << s->lim << "_n" << s->n;
+    {
+        std::map<std::string, int> sizes;
+        for (const auto& g : s->grids) sizes[std::to_string(g.nx) + "x" + std::to_string(g.ny) + "b" +
+                                             std::to_string(g.bc_x) + std::to_string(g.bc_y)] += 1;
+        for (const auto& kv : sizes) key << "_" << kv.first << "*" << kv.second;
+    }
+    if (cache_path) {
+        std::ifstream in(cache_path);
+        std::string k;
+        npde_tuning t = s->tune;
+        while (in >> k >> t.rows_per_task >> t.waves_per_block) {
+            if (k == key.str()) {
+                npde_status st = npde_set_tuning(s, &t);
+                if (st == NPDE_OK && chosen) *chosen = s->tune;
+                return st;
+            }
+        }
+    }
+    // save state and run state
+    size_t total = 0;
+    for (int k = 0; k < s->n; ++k) total += 4 * size_t(s->hdesc[k].plane);
+    double* save = nullptr;
+    if (hipMalloc(&save, total * sizeof(double)) != hipSuccess) return fail(s, NPDE_ERR_NOMEM, "autotune buffer");
+    npde_status st = pull_ctl(s);
+    std::vector<Ctl> ctl_saved = s->hctl;
+    auto copy_state = [&](bool to_save) {
+        size_t off = 0;
+        for (int k = 0; k < s->n; ++k) {
+            const size_t n4 = 4 * size_t(s->hdesc[k].plane);
+            if (to_save) (void)hipMemcpy(save + off, s->hdesc[k].U, n4 * sizeof(double), hipMemcpyDeviceToDevice);
+            else (void)hipMemcpy(s->hdesc[k].U, save + off, n4 * sizeof(double), hipMemcpyDeviceToDevice);
+            off += n4;
+        }
+    };
+    copy_state(true);
+    std::vector<double> dts(s->n);
+    if (st == NPDE_OK) st = npde_compute_dt(s, dts.data(), nullptr);
+    const npde_tuning orig = s->tune;
+    npde_tuning best = orig;
+    double best_ms = 1e30;
+    int max_ny = 1;
+    for (const auto& g : s->grids) max_ny = std::max(max_ny, g.ny);
+    const int rows_c[] = {16, 32, 64, 128, 256};
+    const int wpb_c[] = {1, 2, 4};
+    for (int r : rows_c) {
+        if (st != NPDE_OK) break;
+        if (r > 2 * max_ny && r != rows_c[0]) continue;
+        for (int w : wpb_c) {
+            npde_tuning t = orig;
+            t.rows_per_task = r;
+            t.waves_per_block = w;
+            if ((st = npde_set_tuning(s, &t)) != NPDE_OK) break;
+            double ms_sum = 0.0;
+            for (int rep = 0; rep < 3 && st == NPDE_OK; ++rep) {
+                copy_state(false);
+                st = npde_step(s, dts.data());
+                if (rep > 0) ms_sum += s->last_ms;  // first is warm-up
+            }
+            if (st == NPDE_OK && ms_sum < best_ms) {
+                best_ms = ms_sum;
+                best = t;
+            }
+        }
+    }
+    copy_state(false);
+    (void)hipFree(save);
+    if (st != NPDE_OK) {
+        (void)npde_set_tuning(s, &orig);
+        s->hctl = ctl_saved;
+        (void)push_ctl(s);
+        return st;
+    }
+    st = npde_set_tuning(s, &best);
+    s->hctl = ctl_saved;
+    if (st == NPDE_OK) st = push_ctl(s);
+    if (st == NPDE_OK && cache_path) {
+        std::ofstream out(cache_path, std::ios::app);
+        out << key.str() << " " << best.rows_per_task << " " << best.waves_per_block << "\n";
+    }
+    if (chosen) *chosen = s->tune;
+    return st;
+}
+
+}  // extern "C"

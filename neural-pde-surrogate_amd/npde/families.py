@@ -1,0 +1,105 @@
+"""Seeded initial-condition families of the benchmark workloads.
+
+Every sample is a pure function of (family, index): the generator is numpy's PCG64 seeded with
+[crc32(family), index], the reference's seeding scheme (data/families.py:16-19), so any instance of
+a batch can be regenerated alone.  The distributions are the benchmark's own (they differ from the
+reference's dataset families, whose densities and pressures are log-uniform):
+
+  riemann  four constant quadrants split at a corner (cx, cy) ~ U[0.3, 0.7]^2, each quadrant
+           rho ~ U[0.1, 2], u, v ~ U[-1, 1], p ~ U[0.1, 2]; periodic unit box
+  blast    background rho = 1, u = v = 0, p = 0.1; 1-3 blasts at uniform centres with radius
+           ~ U[0.03, 0.1] and p ~ U[5, 50]; 0-3 density bubbles with radius ~ U[0.05, 0.15] and
+           rho ~ U[0.1, 5]; discs smoothed with tanh edges of width 0.005 on the periodic distance
+           (reference: data/families.py:70-79)
+  kh       two tanh shear layers at y = 1/4, 3/4 with smoothing length 0.025, p = 2.5, density ratio
+           ~ U[1.5, 3], shear du ~ U[0.5, 1.5], v = A sin(4 pi x) with A ~ U[0.005, 0.02]
+
+Each function returns (U, t_end, params) with U the conserved planes (4, n, n).
+"""
+from __future__ import annotations
+
+import zlib
+
+import numpy as np
+
+from .solver import to_conserved
+
+
+def rng_for(family: str, index: int) -> np.random.Generator:
+    return np.random.default_rng([zlib.crc32(family.encode()), int(index)])
+
+
+def cell_centres(n: int) -> np.ndarray:
+    return (np.arange(n) + 0.5) / n
+
+
+def riemann(index: int, n: int, family: str = "riemann_b1"):
+    rng = rng_for(family, index)
+    cx, cy = rng.uniform(0.3, 0.7), rng.uniform(0.3, 0.7)
+    # quadrants: 0 bottom-left, 1 bottom-right, 2 top-left, 3 top-right; columns rho, u, v, p
+    states = np.array([[rng.uniform(0.1, 2.0), rng.uniform(-1.0, 1.0), rng.uniform(-1.0, 1.0),
+                        rng.uniform(0.1, 2.0)] for _ in range(4)])
+    x = cell_centres(n)
+    X, Y = np.meshgrid(x, x)
+    quad = (X > cx).astype(int) + 2 * (Y > cy).astype(int)
+    rho, u, v, p = (states[quad, q] for q in range(4))
+    return to_conserved(rho, u, v, p), 0.5, {"cx": cx, "cy": cy, "states": states}
+
+
+def _periodic_distance(X, Y, cx, cy):
+    dx = (X - cx + 0.5) % 1.0 - 0.5
+    dy = (Y - cy + 0.5) % 1.0 - 0.5
+    return np.sqrt(dx * dx + dy * dy)
+
+
+def _disc(r, radius, width=0.005):
+    return 0.5 * (1.0 - np.tanh((r - radius) / width))
+
+
+BLAST_SIZES = (128, 256, 512, 1024)
+BLAST_WEIGHTS = (4, 3, 2, 1)
+
+
+def blast_size(index: int, family: str = "blast_b2") -> int:
+    rng = rng_for(family + "_size", index)
+    w = np.array(BLAST_WEIGHTS, dtype=float)
+    return int(rng.choice(BLAST_SIZES, p=w / w.sum()))
+
+
+def blast(index: int, n: int | None = None, family: str = "blast_b2"):
+    n = blast_size(index, family) if n is None else n
+    rng = rng_for(family, index)
+    x = cell_centres(n)
+    X, Y = np.meshgrid(x, x)
+    rho = np.ones((n, n))
+    p = np.full((n, n), 0.1)
+    blasts, bubbles = [], []
+    for _ in range(int(rng.integers(1, 4))):
+        cx, cy = rng.uniform(0, 1), rng.uniform(0, 1)
+        r, pb = rng.uniform(0.03, 0.1), rng.uniform(5.0, 50.0)
+        s = _disc(_periodic_distance(X, Y, cx, cy), r)
+        p = p + (pb - p) * s
+        blasts.append((cx, cy, r, pb))
+    for _ in range(int(rng.integers(0, 4))):
+        cx, cy = rng.uniform(0, 1), rng.uniform(0, 1)
+        r, rb = rng.uniform(0.05, 0.15), rng.uniform(0.1, 5.0)
+        s = _disc(_periodic_distance(X, Y, cx, cy), r)
+        rho = rho + (rb - rho) * s
+        bubbles.append((cx, cy, r, rb))
+    z = np.zeros((n, n))
+    return to_conserved(rho, z, z, p), 0.2, {"n": n, "blasts": blasts, "bubbles": bubbles}
+
+
+def kh(index: int, n: int, family: str = "kh_b4"):
+    rng = rng_for(family, index)
+    ratio, du, amp = rng.uniform(1.5, 3.0), rng.uniform(0.5, 1.5), rng.uniform(0.005, 0.02)
+    x = cell_centres(n)
+    X, Y = np.meshgrid(x, x)
+    L = 0.025
+    band = 0.5 * (np.tanh((Y - 0.25) / L) - np.tanh((Y - 0.75) / L))  # ~1 in the middle strip
+    rho = 1.0 + (ratio - 1.0) * band
+    u = 0.5 * du - du * band
+    v = amp * np.sin(4.0 * np.pi * X)
+    p = np.full((n, n), 2.5)
+    t_start = rng.uniform(0.5, 1.5)
+    return to_conserved(rho, u, v, p), 2.0, {"ratio": ratio, "du": du, "amp": amp, "t_start": t_start}

@@ -1,0 +1,232 @@
+"""GPU parity: the HIP path (libnpde.so through its C-ABI) against the oracle and the golden vectors.
+
+Bar: bit-for-bit equality (IEEE ==) of every conserved value, every dt and the step count -- the
+reference solver is binary64 without contraction, and so are the kernels.
+"""
+import numpy as np
+import pytest
+
+from conftest import run_case_grid
+
+import oracle2d
+
+pytestmark = pytest.mark.gpu
+
+SCHEMES = ["rusanov", "hllc"]
+LIMS = ["none", "unlimited", "minmod", "vanleer"]
+RUN_NAMES = ["lax_liu3_64_hllc_vanleer", "lax_liu3_48_rusanov_minmod", "sod_x_64_hllc_unlimited",
+             "sod_y_64_rusanov_vanleer", "sine2d_32_hllc_none", "kh_32_rusanov_vanleer", "kh_40_hllc_minmod",
+             "riemann_48_hllc_vanleer", "riemann_40_hllc_minmod"]
+
+
+def same(a, b):
+    return np.array_equal(np.asarray(a), np.asarray(b), equal_nan=True)
+
+
+def to_npde_grid(npde, g):
+    return npde.make_grid(g.nx, g.ny, g.x_min, g.x_max, g.y_min, g.y_max, g.bc_x, g.bc_y)
+
+
+@pytest.mark.parametrize("name", RUN_NAMES)
+def test_golden_runs_device_loop(npde_mod, golden, name):
+    print(f"case {name}", flush=True)
+    r = golden("runs.npz")
+    g = run_case_grid(r[f"{name}/grid"])
+    _, scheme, lim = (str(x) for x in r[f"{name}/names"])
+    t_end, nout = r[f"{name}/cfg"]
+    with npde_mod.Batch([to_npde_grid(npde_mod, g)], scheme, lim, 0.5) as b:
+        b.set_state(0, r[f"{name}/U0"])
+        b.run_trajectory(float(t_end), int(nout), sizes=())
+        t, steps = b.time(0)
+        U = b.get_state(0)
+        ft = b.frame_times(0)
+    assert steps == int(r[f"{name}/steps"])
+    assert t == float(r[f"{name}/t"])
+    assert same(ft, r[f"{name}/frame_t"])
+    assert same(U, r[f"{name}/U"])
+
+
+@pytest.mark.parametrize("name", ["lax_liu3_64_hllc_vanleer", "sod_y_64_rusanov_vanleer", "riemann_40_hllc_minmod"])
+def test_golden_runs_host_driven_steps(npde_mod, golden, name):
+    """Solver.compute_dt / Solver.step, the reference's call pattern, reproduce the dt sequence."""
+    r = golden("runs.npz")
+    g = run_case_grid(r[f"{name}/grid"])
+    _, scheme, lim = (str(x) for x in r[f"{name}/names"])
+    dts = r[f"{name}/dts"]
+    s = npde_mod.Solver(to_npde_grid(npde_mod, g), scheme, lim, 0.5)
+    U = r[f"{name}/U0"].copy()
+    t_end, nout = r[f"{name}/cfg"]
+    t, k, step = 0.0, 1, 0
+    while step < 25:
+        t_next = 0.0 + (float(t_end) - 0.0) * k / int(nout)
+        if not t < t_next:
+            k += 1
+            continue
+        dt = min(s.compute_dt(U), t_next - t)
+        assert dt == dts[step], step
+        s.step(U, dt)
+        t += dt
+        step += 1
+    s.close()
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("lim", LIMS)
+def test_matrix_against_oracle(npde_mod, oracle, scheme, lim):
+    """Every scheme x limiter on a non-square, mixed-BC grid whose width is not a multiple of the strip."""
+    from npde.families import riemann
+    print(f"case matrix {scheme} {lim}", flush=True)
+    U0, _, _ = riemann(11, 70)
+    U0 = np.ascontiguousarray(U0[:, :53, :])
+    g = oracle2d.Grid.make(70, 53, y=(0.0, 53 / 70), bc_x="transmissive", bc_y="periodic")
+    a = U0.copy()
+    ro = oracle.run(g, scheme, lim, 0.5, a, 0.04, nout=1, record_dts=True)
+    with npde_mod.Batch([to_npde_grid(npde_mod, g)], scheme, lim, 0.5) as b:
+        b.set_state(0, U0)
+        b.advance_to(0.04)
+        t, steps = b.time(0)
+        U = b.get_state(0)
+    assert steps == ro["steps"] and t == ro["t"]
+    assert same(U, a)
+
+
+def test_trajectory_matches_reference_cli(npde_mod, golden):
+    t = golden("traj_ll64.npz")
+    g, t_end, U0 = npde_mod.builtin_problem("lax_liu3", 64)
+    with npde_mod.Batch([g], "hllc", "vanleer", 0.5) as b:
+        b.set_state(0, U0)
+        b.run_trajectory(t_end, 4, sizes=(64, 32, 16))
+        assert same(b.frame_times(0), t["t"])
+        for m in (64, 32, 16):
+            assert same(b.trajectory(0, m), t[f"U_{m}"]), m
+
+
+def test_builtin_problems_match_oracle(npde_mod, oracle):
+    for test, n in [("sod_x", 50), ("sod_y", 30), ("lax_liu3", 40), ("sine2d", 32), ("kh", 48)]:
+        g, t_end, U = npde_mod.builtin_problem(test, n)
+        go, to, Uo = oracle.setup(test, n)
+        assert (g.nx, g.ny, g.bc_x, g.bc_y) == (go.nx, go.ny, go.bc_x, go.bc_y) and t_end == to
+        assert same(U, Uo), test
+
+
+def test_uneven_batch_each_instance_matches_oracle(npde_mod, oracle):
+    """Instances of different sizes and boundaries in one launch; each must equal its own oracle run."""
+    from npde.families import blast, riemann
+    cases = []
+    for k, (nx, ny, bx, by) in enumerate([(64, 64, 1, 1), (128, 32, 1, 0), (37, 53, 0, 0), (96, 96, 1, 1),
+                                          (61, 61, 0, 1), (120, 60, 1, 1)]):
+        U, _, _ = (riemann(k, max(nx, ny)) if k % 2 == 0 else blast(k, max(nx, ny)))
+        cases.append((oracle2d.Grid.make(nx, ny, y=(0.0, ny / nx), bc_x=bx, bc_y=by),
+                      np.ascontiguousarray(U[:, :ny, :nx])))
+    T = 0.02
+    with npde_mod.Batch([to_npde_grid(npde_mod, g) for g, _ in cases], "hllc", "vanleer", 0.5) as b:
+        for k, (_, U) in enumerate(cases):
+            b.set_state(k, U)
+        b.advance_to(T)
+        for k, (g, U) in enumerate(cases):
+            a = U.copy()
+            ro = oracle.run(g, "hllc", "vanleer", 0.5, a, T)
+            t, steps = b.time(k)
+            assert steps == ro["steps"] and t == ro["t"], k
+            assert same(b.get_state(k), a), k
+
+
+@pytest.mark.parametrize("tuning", [dict(rows_per_task=1, waves_per_block=1), dict(rows_per_task=7, waves_per_block=2),
+                                    dict(rows_per_task=64, waves_per_block=4, use_graph=0),
+                                    dict(rows_per_task=200, waves_per_block=4, steps_per_graph=3)])
+def test_launch_shape_does_not_change_bits(npde_mod, oracle, tuning):
+    g, t_end, U0 = npde_mod.builtin_problem("lax_liu3", 96)
+    go, _, _ = oracle.setup("lax_liu3", 96)
+    a = U0.copy()
+    ro = oracle.run(go, "hllc", "vanleer", 0.5, a, 0.05, nout=2)
+    with npde_mod.Batch([g], "hllc", "vanleer", 0.5) as b:
+        b.set_tuning(**tuning)
+        b.set_state(0, U0)
+        b.run_trajectory(0.05, 2, sizes=(48,))
+        assert b.time(0)[1] == ro["steps"]
+        assert same(b.get_state(0), a)
+
+
+def test_nonphysical_state_raises_with_first_bad_cell(npde_mod, golden):
+    r = golden("runs.npz")
+    g, _, U0 = npde_mod.builtin_problem("sod_x", 100)
+    with npde_mod.Batch([g], "rusanov", "unlimited", 0.5) as b:
+        b.set_state(0, U0)
+        with pytest.raises(npde_mod.NonPhysicalStateError) as e:
+            b.advance_to(0.2)
+        assert (e.value.i, e.value.j) == tuple(int(v) for v in r["fail_sod_unlimited/bad"])
+        assert b.time(0)[1] == int(r["fail_sod_unlimited/steps"])
+        assert same(b.get_state(0), r["fail_sod_unlimited/U"])
+
+
+def test_sod_transpose_is_exact(npde_mod):
+    gx, t_end, Ux = npde_mod.builtin_problem("sod_x", 200)
+    gy, _, Uy = npde_mod.builtin_problem("sod_y", 200)
+    with npde_mod.Batch([gx, gy], "hllc", "vanleer", 0.5) as b:
+        b.set_state(0, Ux)
+        b.set_state(1, Uy)
+        b.advance_to(t_end)
+        assert b.time(0)[1] == b.time(1)[1]
+        X, Y = b.get_state(0), b.get_state(1)
+    assert np.abs(X - np.transpose(Y, (0, 2, 1))[[0, 2, 1, 3]]).max() == 0.0
+    assert (X[2] == 0.0).all()
+
+
+def test_lax_liu_256_published_step_count(npde_mod):
+    g, t_end, U0 = npde_mod.builtin_problem("lax_liu3", 256)
+    with npde_mod.Batch([g], "hllc", "vanleer", 0.5) as b:
+        b.set_state(0, U0)
+        b.run_trajectory(t_end, 10, sizes=())
+        assert b.time(0)[1] == 682
+
+
+def test_ghosted_layout_step(npde_mod, oracle):
+    """Solver.step on the reference's ghosted array: interior = oracle step, ghosts = apply_bcs(U^n)."""
+    g, _, U0 = npde_mod.builtin_problem("lax_liu3", 40)
+    go, _, _ = oracle.setup("lax_liu3", 40)
+    A = np.zeros((44, 44, 4))
+    A[2:42, 2:42, :] = np.moveaxis(U0, 0, -1)
+    s = npde_mod.Solver(g, "hllc", "vanleer", 0.5)
+    dt = s.compute_dt(A)
+    dto, _, _, _ = oracle.compute_dt(go, 0.5, U0)
+    assert dt == dto
+    before = npde_mod.apply_bcs(g, A.copy())
+    s.step(A, dt)
+    a = U0.copy()
+    oracle.step(go, "hllc", "vanleer", a, dt)
+    assert same(np.moveaxis(A[2:42, 2:42, :], -1, 0), a)
+    ring = np.ones((44, 44), bool)
+    ring[2:42, 2:42] = False
+    assert same(A[ring], before[ring])
+    s.close()
+
+
+def test_full_size_batch_properties(npde_mod, oracle):
+    """64 x 512^2 random-Riemann batch (the benchmark workload): a few device-loop steps, then
+    (a) instance 0 equals the oracle bit for bit, (b) every instance equals its own single-instance
+    run, (c) periodic totals are conserved to rounding."""
+    from npde.families import riemann
+    n, B, T = 512, 64, 0.004
+    grids = [npde_mod.make_grid(n, n, bc_x=1, bc_y=1)] * B
+    ics = [riemann(k, n)[0] for k in range(B)]
+    with npde_mod.Batch(grids, "hllc", "vanleer", 0.5) as b:
+        for k in range(B):
+            b.set_state(k, ics[k])
+        b.advance_to(T)
+        states = [b.get_state(k) for k in (0, 17, 63)]
+        steps = [b.time(k)[1] for k in (0, 17, 63)]
+        mass = [b.get_state(k)[0].sum() for k in range(0, B, 9)]
+    go = oracle2d.Grid.make(n, n, bc_x="periodic", bc_y="periodic")
+    a = ics[0].copy()
+    ro = oracle.run(go, "hllc", "vanleer", 0.5, a, T)
+    assert steps[0] == ro["steps"]
+    assert same(states[0], a)
+    for idx, k in enumerate((17, 63)):
+        with npde_mod.Batch([grids[0]], "hllc", "vanleer", 0.5) as s:
+            s.set_state(0, ics[k])
+            s.advance_to(T)
+            assert s.time(0)[1] == steps[idx + 1]
+            assert same(s.get_state(0), states[idx + 1])
+    for j, k in enumerate(range(0, B, 9)):
+        m0 = ics[k][0].sum()
+        assert abs(mass[j] - m0) <= 1e-12 * abs(m0)

@@ -1,0 +1,123 @@
+"""CPU tests of the product's host side: the C-ABI library loads and exports every symbol the header
+declares; grid indexing and ghost fills (a port of the reference's test_grid2d.cpp checks); state
+layout handling; seeded workload generators; argument validation without touching a GPU."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def header_functions():
+    text = (ROOT / "include" / "npde.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:npde_status|int|const char\*|void\*)\s+(npde_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    import npde._lib as L
+    lib = L.load()
+    declared = header_functions()
+    assert len(declared) >= 25
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(L.EXPORTED) == declared
+
+
+def test_builtin_problem_on_host_matches_oracle(oracle):
+    """npde_builtin_problem is host code (no GPU): same ICs as the reference's setup()."""
+    import npde
+    for test, n in [("sod_x", 40), ("sod_y", 24), ("lax_liu3", 32), ("sine2d", 20), ("kh", 32)]:
+        g, t_end, U = npde.builtin_problem(test, n)
+        go, to, Uo = oracle.setup(test, n)
+        assert (g.nx, g.ny, t_end) == (go.nx, go.ny, to)
+        assert np.array_equal(U, Uo)
+    with pytest.raises(ValueError):
+        npde.builtin_problem("nope", 8)
+
+
+def test_invalid_arguments_are_rejected_without_a_device():
+    import npde._lib as L
+    lib = L.load()
+    h = C.c_void_p()
+    bad = L.GridDesc(0, 10, 0.0, 1.0, 0.0, 1.0, 0, 0)
+    assert lib.npde_create(C.byref(bad), 1, 3, 0.5, C.byref(h)) == L.ERR_INVALID
+    good = L.GridDesc(8, 8, 0.0, 1.0, 0.0, 1.0, 0, 0)
+    assert lib.npde_create(C.byref(good), 7, 3, 0.5, C.byref(h)) == L.ERR_INVALID   # unknown flux
+    assert lib.npde_create(C.byref(good), 1, 9, 0.5, C.byref(h)) == L.ERR_INVALID   # unknown limiter
+    assert lib.npde_create(C.byref(good), 1, 3, 0.0, C.byref(h)) == L.ERR_INVALID   # cfl <= 0
+    flat = L.GridDesc(8, 8, 1.0, 1.0, 0.0, 1.0, 0, 0)
+    assert lib.npde_create(C.byref(flat), 1, 3, 0.5, C.byref(h)) == L.ERR_INVALID   # empty domain
+
+
+def test_grid_indexing_and_ghost_fills():
+    """test_grid2d.cpp:27-63 on the Python mirror: 5 x 3 grid, tags (i, j) in (mx, my)."""
+    import npde
+    g = npde.make_grid(5, 3, 0.0, 1.0, 0.0, 0.6)
+    assert g.size() == 63 and g.idx(-2, -2) == 0 and g.idx(6, 4) == 62 and g.idx(1, 0) == g.idx(0, 0) + 1
+    assert abs(g.dx - 0.2) < 1e-15 and abs(g.dy - 0.2) < 1e-15
+    assert abs(g.xc(0) - 0.1) < 1e-15 and abs(g.yc(2) - 0.5) < 1e-15
+
+    def tagged(bx, by):
+        gg = npde.make_grid(5, 3, 0.0, 1.0, 0.0, 0.6, bx, by)
+        A = np.zeros((7, 9, 4))
+        for j in range(3):
+            for i in range(5):
+                A[j + 2, i + 2] = [1.0, i, j, 0.0]
+        npde.apply_bcs(gg, A)
+        return lambda i, j: (A[j + 2, i + 2, 1], A[j + 2, i + 2, 2])
+
+    t = tagged(0, 0)
+    assert t(-1, 1) == (0, 1) and t(-2, 1) == (0, 1) and t(5, 2) == (4, 2) and t(6, 2) == (4, 2)
+    assert t(3, -1) == (3, 0) and t(3, -2) == (3, 0) and t(3, 3) == (3, 2) and t(3, 4) == (3, 2)
+    assert t(-2, -2) == (0, 0) and t(6, 4) == (4, 2)
+    p = tagged(1, 1)
+    assert p(-1, 1) == (4, 1) and p(-2, 1) == (3, 1) and p(5, 1) == (0, 1) and p(6, 1) == (1, 1)
+    assert p(2, -1) == (2, 2) and p(2, -2) == (2, 1) and p(2, 3) == (2, 0) and p(2, 4) == (2, 1)
+    assert p(-1, -1) == (4, 2) and p(5, 3) == (0, 0)
+    m = tagged(1, 0)
+    assert m(-1, 1) == (4, 1) and m(2, 3) == (2, 2)
+
+
+def test_conversions_match_reference_known_answers(golden):
+    import npde
+    k = golden("kat.npz")
+    W = k["W"]
+    U = npde.to_conserved(W[:, 0], W[:, 1], W[:, 2], W[:, 3])
+    assert np.array_equal(U.T, k["UL"])
+    rho, u, v, p = npde.to_primitive(k["UL"].T)
+    assert np.array_equal(np.stack([rho, u, v, p], 1), k["prim_UL"])
+
+
+def test_workload_generators_are_seeded_by_family_and_index():
+    from npde import families
+    a, _, pa = families.riemann(5, 64)
+    b, _, pb = families.riemann(5, 64)
+    c, _, _ = families.riemann(6, 64)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    assert 0.3 <= pa["cx"] <= 0.7 and 0.3 <= pa["cy"] <= 0.7
+    st = pa["states"]
+    assert ((st[:, 0] >= 0.1) & (st[:, 0] <= 2)).all() and ((st[:, 3] >= 0.1) & (st[:, 3] <= 2)).all()
+    assert (np.abs(st[:, 1:3]) <= 1).all()
+    sizes = [families.blast_size(k) for k in range(400)]
+    counts = np.array([sizes.count(s) for s in families.BLAST_SIZES]) / 400
+    assert np.allclose(counts, np.array(families.BLAST_WEIGHTS) / 10, atol=0.06)
+    U, t_end, prm = families.blast(3, 64)
+    rho, u, v, p = (U[0], U[1] / U[0], U[2] / U[0], 0.4 * (U[3] - 0.5 * (U[1] ** 2 + U[2] ** 2) / U[0]))
+    assert t_end == 0.2 and (rho > 0).all() and (p > 0).all() and 1 <= len(prm["blasts"]) <= 3
+    U, t_end, prm = families.kh(1, 64)
+    assert t_end == 2.0 and 1.5 <= prm["ratio"] <= 3 and 0.5 <= prm["t_start"] <= 1.5
+
+
+def test_state_layout_detection():
+    import npde
+    from npde.solver import _layout
+    g = npde.make_grid(6, 4)
+    assert _layout(g, np.zeros((4, 4, 6))) == 0
+    assert _layout(g, np.zeros((8, 10, 4))) == 1
+    with pytest.raises(ValueError):
+        _layout(g, np.zeros((4, 6, 4)))
+    with pytest.raises(TypeError):
+        _layout(g, np.zeros((4, 4, 6), dtype=np.float32))

@@ -1,0 +1,209 @@
+"""Pins the CPU oracle (oracle/euler2d_oracle.cpp) before it is trusted as the GPU's checker.
+
+1. Golden vectors written by the reference's own code (tests/golden/make_golden.py): point-wise
+   known answers, whole runs with their dt sequences, a non-physical failure, a CLI trajectory.
+2. The reference build itself (oracle/_ref), where present, on inputs the fixtures do not hold.
+3. What the mathematics fixes: consistency F(U, U) = F(U), rotation symmetry of the y faces, exact
+   x/y transposition of Sod, free-stream preservation, conservation, the exact Sod plateau, the
+   published convergence table and step counts, and closed forms of the limiters.
+"""
+import numpy as np
+import pytest
+
+from conftest import run_case_grid
+
+import oracle2d
+
+SCHEMES = ["rusanov", "hllc"]
+LIMS = ["none", "unlimited", "minmod", "vanleer"]
+
+
+def same(a, b):
+    """IEEE equality element by element (+0 == -0); NaN matches NaN (failed states hold NaNs)"""
+    return np.array_equal(np.asarray(a), np.asarray(b), equal_nan=True)
+
+
+# ---------------------------------------------------------------- 1. golden vectors
+def test_pointwise_known_answers(oracle, golden):
+    k = golden("kat.npz")
+    assert same(oracle.to_conserved(k["W"]), k["UL"])
+    assert same(oracle.to_conserved(k["W2"]), k["UR"])
+    assert same(oracle.to_primitive(k["UL"]), k["prim_UL"])
+    assert same(oracle.sound_speed(k["W"]), k["sound_W"])
+    assert same(oracle.physical_flux(k["UL"]), k["flux_x_UL"])
+    for lim in LIMS:
+        assert same(oracle.limiter(lim, k["lim_a"], k["lim_b"]), k[f"lim_{lim}"]), lim
+    for s in SCHEMES:
+        for d in (0, 1):
+            assert same(oracle.face_flux(s, d, k["UL"], k["UR"]), k[f"{s}_{'xy'[d]}"]), (s, d)
+
+
+RUN_NAMES = ["lax_liu3_64_hllc_vanleer", "lax_liu3_48_rusanov_minmod", "sod_x_64_hllc_unlimited",
+             "sod_y_64_rusanov_vanleer", "sine2d_32_hllc_none", "kh_32_rusanov_vanleer", "kh_40_hllc_minmod",
+             "riemann_48_hllc_vanleer", "riemann_40_hllc_minmod"]
+
+
+@pytest.mark.parametrize("name", RUN_NAMES)
+def test_whole_runs_bitwise(oracle, golden, name):
+    r = golden("runs.npz")
+    g = run_case_grid(r[f"{name}/grid"])
+    test, scheme, lim = (str(x) for x in r[f"{name}/names"])
+    t_end, nout = r[f"{name}/cfg"]
+    U = r[f"{name}/U0"].copy()
+    out = oracle.run(g, scheme, lim, 0.5, U, float(t_end), nout=int(nout), record_dts=True, max_steps=100000)
+    assert out["rc"] == 0
+    assert out["steps"] == int(r[f"{name}/steps"])
+    assert same(out["dts"], r[f"{name}/dts"])
+    assert out["t"] == float(r[f"{name}/t"])
+    assert same(out["frame_t"], r[f"{name}/frame_t"])
+    assert same(U, r[f"{name}/U"])
+
+
+def test_builtin_initial_conditions(oracle, golden):
+    r = golden("runs.npz")
+    for name in RUN_NAMES:
+        test = str(r[f"{name}/names"][0])
+        if test == "riemann":
+            continue
+        n = int(r[f"{name}/grid"][0] if test != "sod_y" else r[f"{name}/grid"][1])
+        g, _, U0 = oracle.setup(test, n)
+        assert same(U0, r[f"{name}/U0"]), name
+
+
+def test_nonphysical_state_is_reported(oracle, golden):
+    r = golden("runs.npz")
+    g, _, U0 = oracle.setup("sod_x", 100)
+    assert same(U0, r["fail_sod_unlimited/U0"])
+    U = U0.copy()
+    out = oracle.run(g, "rusanov", "unlimited", 0.5, U, 0.2, nout=1)
+    assert out["rc"] == 1
+    assert out["bad"] == tuple(int(v) for v in r["fail_sod_unlimited/bad"])
+    assert out["steps"] == int(r["fail_sod_unlimited/steps"])
+    assert same(U, r["fail_sod_unlimited/U"])
+
+
+def test_trajectory_block_averages_match_reference_cli(oracle, golden):
+    t = golden("traj_ll64.npz")
+    g, t_end, U = oracle.setup("lax_liu3", 64)
+    out = oracle.run(g, "hllc", "vanleer", 0.5, U, t_end, nout=4, sizes=(64, 32, 16))
+    assert same(out["frame_t"], t["t"])
+    for m in (64, 32, 16):
+        assert out["frames"][m].dtype == np.float32
+        assert same(out["frames"][m], t[f"U_{m}"]), m
+
+
+def test_published_step_count_lax_liu_256(oracle):
+    """experiments/results.md:92-94: 682 steps at 256^2, t = 0.3, nout = 10."""
+    g, t_end, U = oracle.setup("lax_liu3", 256)
+    out = oracle.run(g, "hllc", "vanleer", 0.5, U, t_end, nout=10)
+    assert out["steps"] == 682
+
+
+# ---------------------------------------------------------------- 2. the reference build itself
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("lim", LIMS)
+def test_against_reference_build_random_riemann(oracle, reference, scheme, lim):
+    from npde.families import riemann
+    g = oracle2d.Grid.make(36, 28, y=(0.0, 28 / 36), bc_x="periodic", bc_y="transmissive")
+    U0, _, _ = riemann(3, 36)
+    U0 = np.ascontiguousarray(U0[:, :28, :])
+    a, b = U0.copy(), U0.copy()
+    ra = oracle.run(g, scheme, lim, 0.5, a, 0.05, nout=2, record_dts=True)
+    rb = reference.run(g, scheme, lim, 0.5, b, 0.05, nout=2, record_dts=True)
+    assert ra["rc"] == rb["rc"]
+    assert ra["steps"] == rb["steps"]
+    assert same(ra["dts"], rb["dts"])
+    assert same(a, b)
+
+
+def test_against_reference_build_pointwise_random(oracle, reference):
+    rng = np.random.default_rng(5)
+    n = 20000
+    W = np.stack([rng.uniform(1e-3, 10, n), rng.normal(0, 3, n), rng.normal(0, 3, n), rng.uniform(1e-3, 10, n)], 1)
+    W2 = np.stack([rng.uniform(1e-3, 10, n), rng.normal(0, 3, n), rng.normal(0, 3, n), rng.uniform(1e-3, 10, n)], 1)
+    UL, UR = reference.to_conserved(W), reference.to_conserved(W2)
+    for s in SCHEMES:
+        for d in (0, 1):
+            assert same(oracle.face_flux(s, d, UL, UR), reference.face_flux(s, d, UL, UR))
+
+
+# ---------------------------------------------------------------- 3. what the mathematics fixes
+def test_limiter_closed_forms(oracle):
+    a = np.array([1.0, 2.0, -1.0, 1.0, 0.0, 3.0, -2.0, 0.5])
+    b = np.array([2.0, 1.0, 1.0, -1.0, 1.0, 3.0, -4.0, 0.5])
+    assert same(oracle.limiter("none", a, b), np.zeros(8))
+    assert same(oracle.limiter("unlimited", a, b), 0.5 * (a + b))
+    assert same(oracle.limiter("minmod", a, b), [1.0, 1.0, 0.0, 0.0, 0.0, 3.0, -2.0, 0.5])
+    # van Leer: harmonic mean 2ab/(a+b) for same signs, else 0 (TVD); symmetric in (a, b)
+    vl = oracle.limiter("vanleer", a, b)
+    assert np.allclose(vl, [4 / 3, 4 / 3, 0.0, 0.0, 0.0, 3.0, -8 / 3, 0.5], rtol=0, atol=1e-15)
+    assert same(oracle.limiter("vanleer", b, a), vl)
+
+
+def test_flux_consistency_and_rotation(oracle):
+    """test_riemann2d.cpp:42-47 (HLLC(U,U) = F(U)) and :33-39 (y face = x face with momenta swapped)."""
+    W = np.array([[1.3, 0.4, -0.7, 2.1], [0.2, 3.0, 0.1, 0.05], [2.0, -1.5, 0.7, 4.0]])
+    U = oracle.to_conserved(W)
+    F = oracle.physical_flux(U)
+    for s in SCHEMES:
+        got = oracle.face_flux(s, 0, U, U)
+        assert np.allclose(got, F, rtol=1e-13, atol=1e-13), s
+    sw = lambda X: X[:, [0, 2, 1, 3]]
+    rng = np.random.default_rng(2)
+    Wl = np.stack([rng.uniform(0.1, 2, 50), rng.normal(size=50), rng.normal(size=50), rng.uniform(0.1, 2, 50)], 1)
+    Wr = np.stack([rng.uniform(0.1, 2, 50), rng.normal(size=50), rng.normal(size=50), rng.uniform(0.1, 2, 50)], 1)
+    UL, UR = oracle.to_conserved(Wl), oracle.to_conserved(Wr)
+    for s in SCHEMES:
+        assert same(oracle.face_flux(s, 1, sw(UL), sw(UR)), sw(oracle.face_flux(s, 0, UL, UR)))
+
+
+def test_sod_transpose_symmetry_conservation_and_plateau(oracle):
+    """test_solver2d.cpp:61-108 restated on the oracle."""
+    gx, t_end, Ux = oracle.setup("sod_x", 200)
+    gy, _, Uy = oracle.setup("sod_y", 200)
+    mass0 = Ux[0].sum() * (1 / 200) * (4 / 200)
+    en0 = Ux[3].sum() * (1 / 200) * (4 / 200)
+    rx = oracle.run(gx, "hllc", "vanleer", 0.5, Ux, t_end)
+    ry = oracle.run(gy, "hllc", "vanleer", 0.5, Uy, t_end)
+    assert rx["steps"] == ry["steps"]
+    assert (Ux[:, 1:, :] == Ux[:, :1, :]).all()            # rows identical
+    assert (Ux[2] == 0.0).all()                              # no y-momentum created
+    T = np.transpose(Uy, (0, 2, 1))[[0, 2, 1, 3]]            # y-run transposed, momenta swapped
+    assert np.abs(Ux - T).max() == 0.0
+    assert abs(Ux[0].sum() * (1 / 200) * (4 / 200) - mass0) < 1e-13
+    assert abs(Ux[3].sum() * (1 / 200) * (4 / 200) - en0) < 1e-13
+    i = int(0.78 * 200)
+    rho, mx, _, E = Ux[:, 0, i]
+    u = mx / rho
+    p = 0.4 * (E - 0.5 * rho * u * u)
+    assert abs(rho - 0.26557) < 1e-3 and abs(p - 0.30313) < 1e-3 and abs(u - 0.92745) < 1e-3
+
+
+def test_free_stream_preserved(oracle):
+    """test_solver2d.cpp:36-53: a uniform moving state stays uniform on a periodic 16 x 12 grid."""
+    g = oracle2d.Grid.make(16, 12, bc_x="periodic", bc_y="periodic")
+    U0 = oracle.to_conserved(np.array([[1.3, 0.7, -0.4, 2.0]]))[0]
+    U = np.ascontiguousarray(np.broadcast_to(U0[:, None, None], (4, 12, 16)))
+    oracle.run(g, "hllc", "vanleer", 0.5, U, 0.5)
+    assert np.abs(U - U0[:, None, None]).max() < 1e-13
+
+
+@pytest.mark.parametrize("n,lim,l1", [(32, "vanleer", 1.088e-2), (64, "vanleer", 3.203e-3),
+                                      (32, "none", 9.024e-2), (64, "none", 5.862e-2)])
+def test_published_convergence_table(oracle, n, lim, l1):
+    """experiments/results.md:75-80 (L1 density error of sine2d at t = 1, printed to 4 digits)."""
+    g, t_end, U = oracle.setup("sine2d", n)
+    out = oracle.run(g, "hllc", lim, 0.5, U, t_end, nout=1)
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    exact = 1.0 + 0.2 * np.sin(2 * np.pi * (X + Y - 2 * out["t"]))
+    err = np.abs(U[0] - exact).sum() * (1 / n) * (1 / n)
+    assert float(f"{err:.3e}") == l1
+
+
+def test_lax_liu_near_symmetry(oracle):
+    """README.md:64: Lax-Liu 3 is symmetric about x = y to ~3e-14 (x-then-y accumulation breaks exactness)."""
+    g, t_end, U = oracle.setup("lax_liu3", 64)
+    oracle.run(g, "hllc", "vanleer", 0.5, U, t_end)
+    d = np.abs(U[0] - U[0].T).max()
+    assert 0.0 < d < 1e-12
