@@ -113,6 +113,13 @@ npde_status npde_set_flux_correction(npde_solver* s, int instance, const double*
  * with U = NULL to learn the grid size. */
 npde_status npde_builtin_problem(const char* test, int n, npde_grid_desc* grid, double* t_end, double* U);
 
+/* ---- diagnostics (no counterpart in the reference) ---- */
+/* Element-wise on the device: q_fast = x / y by the kernels' shared-reciprocal sequence, q_ieee = x / y
+ * by the compiler's division, root = sqrt(x); host arrays of length n.  Lets a test prove, on the GPU
+ * it runs on, that both quotients are the correctly rounded IEEE ones. */
+npde_status npde_selftest_arith(const double* x, const double* y, long long n, double* q_fast, double* q_ieee,
+                                double* root);
+
 /* ---- performance ---- */
 npde_status npde_get_tuning(const npde_solver* s, npde_tuning* t);
 npde_status npde_set_tuning(npde_solver* s, const npde_tuning* t);

@@ -51,7 +51,8 @@ __device__ __forceinline__ Prim from_next(const Prim& w) {
 }
 __device__ __forceinline__ Side from_prev(const Side& s) {
     return {from_prev(s.d),  from_prev(s.mn), from_prev(s.mt), from_prev(s.e),
-            from_prev(s.un), from_prev(s.ut), from_prev(s.p),  from_prev(s.a)};
+            from_prev(s.un), from_prev(s.ut), from_prev(s.p),  from_prev(s.a),
+            from_prev(s.rd)};
 }
 __device__ __forceinline__ Cons from_next(const Cons& c) {
     return {from_next(c.d), from_next(c.m), from_next(c.n), from_next(c.e)};
@@ -294,9 +295,10 @@ __global__ __launch_bounds__(256) void k_stage(const InstDesc* __restrict__ desc
         if (owner) store4(O, (long long)f * nx + ci, P, out);
 
         if constexpr (STAGE == 2) {  // wave speeds of U^{n+1} for the next dt (solver2d.hpp:67-83)
-            const Prim w = prim_of(out);
+            const double rd = recip(out.d);
+            const Prim w = prim_of(out.d, out.m, out.n, out.e, rd);
             const bool badc = !(w.d > 0.0) || !(w.p > 0.0);
-            const double a = sound(w.d, w.p);
+            const double a = sound(w.d, w.p, rd);
             if (owner) {
                 if (badc) {
                     const int idx = f * nx + i;
@@ -340,9 +342,11 @@ __global__ __launch_bounds__(256) void k_wavespeed(const InstDesc* __restrict__ 
     double sx = 0.0, sy = 0.0;
     int bad = kNoBad;
     for (int f = tk.j0; f < tk.j1; ++f) {
-        const Prim w = prim_of(load4(D.U, (long long)f * nx + ci, P));
+        const Cons u = load4(D.U, (long long)f * nx + ci, P);
+        const double rd = recip(u.d);
+        const Prim w = prim_of(u.d, u.m, u.n, u.e, rd);
         const bool badc = !(w.d > 0.0) || !(w.p > 0.0);
-        const double a = sound(w.d, w.p);
+        const double a = sound(w.d, w.p, rd);
         if (owner) {
             if (badc) {
                 const int idx = f * nx + i;
@@ -444,6 +448,24 @@ __global__ void k_prep_host_step(Ctl* __restrict__ ctl, int n, const double* __r
     ctl[k].mode = kModeHost;
     ctl[k].dt = dt;
     ctl[k].active = dt >= 0.0 ? 1 : 0;
+}
+
+// Arithmetic self-test: the shared-reciprocal quotient, the plain IEEE quotient and sqrt, element-wise.
+__global__ void k_selftest_arith(const double* __restrict__ x, const double* __restrict__ y, long long n,
+                                 double* __restrict__ q_fast, double* __restrict__ q_ieee,
+                                 double* __restrict__ root) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double a = x[k], b = y[k];
+    q_fast[k] = div_r(a, b, recip(b));
+    q_ieee[k] = a / b;
+    root[k] = dsqrt(a);
+}
+
+hipError_t launch_selftest_arith(const double* x, const double* y, long long n, double* qf, double* qi, double* rt,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL(k_selftest_arith, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, x, y, n, qf, qi, rt);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------------------------

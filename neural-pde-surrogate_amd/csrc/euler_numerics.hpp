@@ -36,8 +36,8 @@ struct Cons { double d, m, n, e; };  // rho, rho*u, rho*v, E
 struct Prim { double d, u, v, p; };  // rho, u, v, p
 
 // One side of a face in the face-normal frame: conserved (d, mn, mt, e), the primitives recovered
-// from them (un, ut, p) and the sound speed a.
-struct Side { double d, mn, mt, e, un, ut, p, a; };
+// from them (un, ut, p), the sound speed a, and the reciprocal rd of d used by its divisions.
+struct Side { double d, mn, mt, e, un, ut, p, a, rd; };
 struct Flux { double d, mn, mt, e; };
 
 NPDE_HD double dsqrt(double x) {
@@ -53,19 +53,54 @@ NPDE_HD double dabs(double x) { return __builtin_fabs(x); }
 NPDE_HD double tmin(double a, double b) { return (b < a) ? b : a; }
 NPDE_HD double tmax(double a, double b) { return (a < b) ? b : a; }
 
-NPDE_HD Prim prim_of(double d, double m, double n, double e) {
-    const double u = m / d;
-    const double v = n / d;
+// ---- correctly rounded division with a shared reciprocal -----------------------------------
+// gfx950's binary64 division x / y is: r = rcp(y) refined by two Newton steps (4 FMAs), q0 = x*r,
+// one FMA remainder and one FMA correction, wrapped in div_scale / div_fmas / div_fixup, which only
+// rescale operands whose exponents are near the ends of the range and patch zeros, infinities and
+// NaNs.  Several of the scheme's divisions share a denominator (rho in to_primitive and in the sound
+// speed; the constant gamma - 1 in to_conserved), so the reciprocal part is run once per
+// denominator (`recip`) and each quotient costs three instructions (`div_r`).  For operands where
+// the rescaling does not trigger -- |x| and |y| in [2^-900, 2^700] or x = 0, i.e. every value a
+// physical state produces -- this is the compiler's own instruction sequence, hence the same IEEE
+// quotient; the remainder is taken as y*q0 - x so that x = -0 still gives -0.
+// Define NPDE_IEEE_DIV to use the plain operator everywhere.
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(NPDE_IEEE_DIV)
+NPDE_HD double recip(double y) {
+    double r = __builtin_amdgcn_rcp(y);
+    double e = __builtin_fma(-y, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-y, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+NPDE_HD double div_r(double x, double y, double r) {
+    const double q0 = x * r;
+    const double rem = __builtin_fma(y, q0, -x);
+    return __builtin_fma(-rem, r, q0);
+}
+#else
+NPDE_HD double recip(double y) { return y; }  // the plain path divides by y itself
+NPDE_HD double div_r(double x, double y, double) { return x / y; }
+#endif
+
+// reciprocal of gamma - 1 for div_r(x, kGm1, rgm1()); a pure function, hoisted by the compiler
+NPDE_HD double rgm1() { return recip(kGm1); }
+
+NPDE_HD Prim prim_of(double d, double m, double n, double e, double rd) {
+    const double u = div_r(m, d, rd);
+    const double v = div_r(n, d, rd);
     const double kin = (0.5 * d) * (u * u + v * v);
     return {d, u, v, kGm1 * (e - kin)};
 }
 
+NPDE_HD Prim prim_of(double d, double m, double n, double e) { return prim_of(d, m, n, e, recip(d)); }
+
 NPDE_HD Cons cons_of(const Prim& w) {
     const double kin = (0.5 * w.d) * (w.u * w.u + w.v * w.v);
-    return {w.d, w.d * w.u, w.d * w.v, w.p / kGm1 + kin};
+    return {w.d, w.d * w.u, w.d * w.v, div_r(w.p, kGm1, rgm1()) + kin};
 }
 
-NPDE_HD double sound(double d, double p) { return dsqrt((kGamma * p) / d); }
+NPDE_HD double sound(double d, double p, double rd) { return dsqrt(div_r(kGamma * p, d, rd)); }
+NPDE_HD double sound(double d, double p) { return sound(d, p, recip(d)); }
 
 template <int LIM>
 NPDE_HD double limit(double a, double b) {
@@ -94,12 +129,13 @@ NPDE_HD Prim slope(const Prim& m, const Prim& c, const Prim& p) {
 NPDE_HD Side edge_side(const Prim& w, const Prim& dw, double h) {
     const Prim we{w.d + h * dw.d, w.u + h * dw.u, w.v + h * dw.v, w.p + h * dw.p};
     const Cons c = cons_of(we);
-    const Prim r = prim_of(c.d, c.m, c.n, c.e);
-    return {c.d, c.m, c.n, c.e, r.u, r.v, r.p, sound(r.d, r.p)};
+    const double rd = recip(c.d);
+    const Prim r = prim_of(c.d, c.m, c.n, c.e, rd);
+    return {c.d, c.m, c.n, c.e, r.u, r.v, r.p, sound(r.d, r.p, rd), rd};
 }
 
 // x frame -> y frame (normal = y): relabel momenta and velocities
-NPDE_HD Side to_y_frame(const Side& s) { return {s.d, s.mt, s.mn, s.e, s.ut, s.un, s.p, s.a}; }
+NPDE_HD Side to_y_frame(const Side& s) { return {s.d, s.mt, s.mn, s.e, s.ut, s.un, s.p, s.a, s.rd}; }
 
 NPDE_HD Flux phys_flux(const Side& s) {
     return {s.mn, s.mn * s.un + s.p, s.mt * s.un, s.un * (s.e + s.p)};
@@ -107,7 +143,8 @@ NPDE_HD Flux phys_flux(const Side& s) {
 
 NPDE_HD Side pick(bool first, const Side& a, const Side& b) {
     return {first ? a.d : b.d,   first ? a.mn : b.mn, first ? a.mt : b.mt, first ? a.e : b.e,
-            first ? a.un : b.un, first ? a.ut : b.ut, first ? a.p : b.p,   first ? a.a : b.a};
+            first ? a.un : b.un, first ? a.ut : b.ut, first ? a.p : b.p,   first ? a.a : b.a,
+            first ? a.rd : b.rd};
 }
 
 // HLLC with Davis wave speeds (riemann2d.hpp:18-39).  The reference's four-way branch
@@ -128,7 +165,7 @@ NPDE_HD Flux hllc(const Side& A, const Side& B) {
     const double q = X.d * (S - X.un);
     const double f = q / (S - ss);
     const double sd = f, smn = f * ss, smt = f * X.ut;
-    const double se = f * (X.e / X.d + (ss - X.un) * (ss + X.p / q));
+    const double se = f * (div_r(X.e, X.d, X.rd) + (ss - X.un) * (ss + X.p / q));
     const Flux Fs{F.d + S * (sd - X.d), F.mn + S * (smn - X.mn), F.mt + S * (smt - X.mt),
                   F.e + S * (se - X.e)};
     return {star ? Fs.d : F.d, star ? Fs.mn : F.mn, star ? Fs.mt : F.mt, star ? Fs.e : F.e};

@@ -30,6 +30,8 @@ hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int
                         hipStream_t s);
 hipError_t launch_block_average(const InstDesc* desc, int inst, int m, int nyc, float* out, hipStream_t s);
 hipError_t launch_prep_host_step(Ctl* ctl, int n, const double* dts, hipStream_t s);
+hipError_t launch_selftest_arith(const double* x, const double* y, long long n, double* qf, double* qi, double* rt,
+                                 hipStream_t s);
 }  // namespace npde
 
 using namespace npde;
@@ -713,6 +715,23 @@ npde_status npde_last_elapsed_ms(const npde_solver* s, double* ms) {
 void* npde_stream(npde_solver* s) { return s ? (void*)s->stream : nullptr; }
 
 // Built-in initial conditions of the driver (main2d.cpp:52-135), evaluated with the host libm.
+npde_status npde_selftest_arith(const double* x, const double* y, long long n, double* q_fast, double* q_ieee,
+                                double* root) {
+    if (!x || !y || !q_fast || !q_ieee || !root || n <= 0) return NPDE_ERR_INVALID;
+    double* d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * 5 * size_t(n)) != hipSuccess) return NPDE_ERR_NOMEM;
+    const size_t b = sizeof(double) * size_t(n);
+    hipError_t e = hipMemcpy(d, x, b, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + n, y, b, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_selftest_arith(d, d + n, n, d + 2 * n, d + 3 * n, d + 4 * n, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(q_fast, d + 2 * n, b, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(q_ieee, d + 3 * n, b, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(root, d + 4 * n, b, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e == hipSuccess ? NPDE_OK : NPDE_ERR_DEVICE;
+}
+
 npde_status npde_builtin_problem(const char* test, int n, npde_grid_desc* grid, double* t_end, double* U) {
     if (!test || !grid || !t_end || n <= 0) return NPDE_ERR_INVALID;
     const std::string name(test);

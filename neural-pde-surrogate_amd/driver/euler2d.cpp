@@ -1,0 +1,252 @@
+// npde-euler2d: drop-in for the reference's euler2d executable, running the hot path on the GPU.
+//
+// Same flags and defaults (main2d.cpp:24-43), built-in tests (main2d.cpp:52-135, via
+// npde_builtin_problem), --init IC files, snapshot and trajectory output modes with the same file
+// layouts and names (main2d.cpp:192-218), the same stdout lines and the "Done:" throughput line that
+// scripts parse (main2d.cpp:189-190, 245, 257-260), and exit code 1 with the reference's messages on
+// bad arguments, bad files and non-physical states (solver2d.hpp:116-126).
+//
+// The time loop (main2d.cpp:221-246) runs device-resident: a trajectory run is one
+// npde_run_trajectory call (frames block-averaged on the GPU as each target is reached); snapshot
+// mode advances frame by frame with npde_advance_to and writes each state.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <filesystem>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/npde.h"
+#include "h5io.hpp"
+
+namespace {
+
+std::map<std::string, std::string> parse_args(int argc, char* argv[]) {
+    std::map<std::string, std::string> a = {
+        {"test", "lax_liu3"}, {"n", "256"}, {"scheme", "hllc"}, {"limiter", "vanleer"}, {"cfl", "0.5"},
+        {"nout", "10"}, {"out", "runs"}, {"tend", "default"}, {"profile", "no"}, {"init", ""},
+        {"traj", ""}, {"save", ""}, {"tstart", "0"}};
+    for (int k = 1; k < argc; ++k) {
+        std::string key = argv[k];
+        if (key.rfind("--", 0) != 0 || k + 1 >= argc) {
+            std::cerr << "Expected '--key value', got '" << key << "'\n";
+            std::exit(1);
+        }
+        key = key.substr(2);
+        if (!a.count(key)) {
+            std::cerr << "Unknown option --" << key << '\n';
+            std::exit(1);
+        }
+        a[key] = argv[++k];
+    }
+    return a;
+}
+
+std::vector<int> parse_sizes(const std::string& s) {
+    std::vector<int> out;
+    size_t start = 0;
+    while (start < s.size()) {
+        size_t end = s.find(',', start);
+        if (end == std::string::npos) end = s.size();
+        out.push_back(std::stoi(s.substr(start, end - start)));
+        start = end + 1;
+    }
+    return out;
+}
+
+[[noreturn]] void die_nonphysical(const npde_bad_cell& b) {
+    std::cerr << "Non-physical state in cell (" << b.i << ", " << b.j << "): rho = " << b.rho << ", p = " << b.p
+              << '\n';
+    std::exit(1);
+}
+
+void check(npde_solver* s, npde_status st, const npde_bad_cell& bad) {
+    if (st == NPDE_OK) return;
+    if (st == NPDE_ERR_NONPHYSICAL) die_nonphysical(bad);
+    std::cerr << "npde error " << int(st) << ": " << npde_last_error(s) << '\n';
+    std::exit(1);
+}
+
+}  // namespace
+
+int main(int argc, char* argv[]) {
+    auto a = parse_args(argc, argv);
+    const std::string test = a["test"], scheme = a["scheme"], lim_name = a["limiter"];
+    const int n = std::atoi(a["n"].c_str()), nout = std::atoi(a["nout"].c_str());
+    const double cfl = std::atof(a["cfl"].c_str());
+    if (n <= 0 || nout <= 0 || !(cfl > 0.0)) {
+        std::cerr << "n, nout and cfl must be positive\n";
+        return 1;
+    }
+    int flux;
+    if (scheme == "rusanov") flux = NPDE_FLUX_RUSANOV;
+    else if (scheme == "hllc") flux = NPDE_FLUX_HLLC;
+    else {
+        std::cerr << "Unknown scheme '" << scheme << "' (use rusanov or hllc)\n";
+        return 1;
+    }
+    int lim;
+    if (lim_name == "none") lim = NPDE_LIM_NONE;
+    else if (lim_name == "minmod") lim = NPDE_LIM_MINMOD;
+    else if (lim_name == "vanleer") lim = NPDE_LIM_VANLEER;
+    else if (lim_name == "unlimited") lim = NPDE_LIM_UNLIMITED;
+    else {
+        std::cerr << "Unknown limiter '" << lim_name << "'\n";
+        return 1;
+    }
+
+    npde_grid_desc g{};
+    double t_end = 0.0;
+    std::vector<double> U;
+    std::string run_name = test;
+    if (!a["init"].empty()) {
+        npde_io::InitInfo ii;
+        try {
+            ii = npde_io::read_initial_condition(a["init"], g, U);
+        } catch (const std::exception& e) {
+            std::cerr << "Error reading " << a["init"] << ": " << e.what() << '\n';
+            return 1;
+        }
+        t_end = ii.t_end;
+        run_name = ii.test;
+        if (t_end <= 0.0 && a["tend"] == "default") {
+            std::cerr << "The initial-condition file has no t_end; give one with --tend\n";
+            return 1;
+        }
+    } else {
+        if (npde_builtin_problem(test.c_str(), n, &g, &t_end, nullptr) != NPDE_OK) {
+            std::cerr << "Unknown test '" << test << "' (use sod_x, sod_y, lax_liu3, sine2d or kh)\n";
+            return 1;
+        }
+        U.assign(size_t(4) * g.nx * g.ny, 0.0);
+        npde_builtin_problem(test.c_str(), n, &g, &t_end, U.data());
+    }
+    if (a["tend"] != "default") t_end = std::atof(a["tend"].c_str());
+    if (!(t_end > 0.0)) {
+        std::cerr << "tend must be positive\n";
+        return 1;
+    }
+
+    npde_solver* s = nullptr;
+    if (npde_create(&g, flux, lim, cfl, &s) != NPDE_OK) {
+        std::cerr << "could not create the GPU solver (is a GPU visible?)\n";
+        return 1;
+    }
+    std::unique_ptr<npde_solver, npde_status (*)(npde_solver*)> guard(s, npde_destroy);
+    npde_io::RunInfo info{run_name, scheme, lim_name, cfl, 0};
+    std::printf("%s: %d x %d cells, t_end = %g, %s + %s, CFL %g, 1 GPU\n", run_name.c_str(), g.nx, g.ny, t_end,
+                scheme.c_str(), lim_name.c_str(), cfl);
+
+    const bool traj_mode = !a["traj"].empty();
+    std::vector<int> sizes;
+    std::string dir;
+    if (traj_mode) {
+        try {
+            sizes = a["save"].empty() ? std::vector<int>{g.nx} : parse_sizes(a["save"]);
+            for (int m : sizes) {
+                if (m <= 0 || g.nx % m != 0)
+                    throw std::runtime_error("save size " + std::to_string(m) + " does not divide nx = " +
+                                             std::to_string(g.nx));
+                if (g.ny % (g.nx / m) != 0)
+                    throw std::runtime_error("save size " + std::to_string(m) + " does not divide ny evenly");
+            }
+            if (sizes.size() > 4) throw std::runtime_error("at most 4 save sizes");
+        } catch (const std::exception& e) {
+            std::cerr << "Bad --save: " << e.what() << '\n';
+            return 1;
+        }
+    } else {
+        dir = a["out"] + "/" + run_name + "_" + scheme + "_" + lim_name + "_N" + std::to_string(g.nx);
+        std::filesystem::create_directories(dir);
+    }
+    auto snap_path = [&](int k) {
+        char buf[32];
+        std::snprintf(buf, sizeof buf, "/snap_%04d.h5", k);
+        return dir + buf;
+    };
+
+    const double t_start = std::atof(a["tstart"].c_str());
+    if (!(t_start >= 0.0 && t_start < t_end)) {
+        std::cerr << "need 0 <= tstart < t_end\n";
+        return 1;
+    }
+    npde_bad_cell bad{};
+    check(s, npde_set_state(s, 0, U.data(), NPDE_LAYOUT_PLANES), bad);
+
+    double t = 0.0;
+    long long steps = 0;
+    double device_ms = 0.0, output_s = 0.0;
+    auto t0 = std::chrono::steady_clock::now();
+    auto lap = [](std::chrono::steady_clock::time_point s0) {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - s0).count();
+    };
+    if (traj_mode) {
+        check(s, npde_run_trajectory(s, t_start, t_end, nout, sizes.data(), int(sizes.size()), &bad), bad);
+        npde_last_elapsed_ms(s, &device_ms);
+        npde_get_time(s, 0, &t, &steps);
+        auto o0 = std::chrono::steady_clock::now();
+        std::vector<double> times(nout + 1);
+        check(s, npde_get_frame_times(s, 0, times.data()), bad);
+        std::vector<npde_io::Level> levels;
+        for (int m : sizes) {
+            const int nyc = g.ny / (g.nx / m);
+            npde_io::Level L{m, nyc, std::vector<float>(size_t(nout + 1) * 4 * nyc * m)};
+            check(s, npde_get_trajectory(s, 0, m, L.data.data()), bad);
+            levels.push_back(std::move(L));
+        }
+        info.step = int(steps);
+        try {
+            npde_io::write_trajectory(a["traj"], g, times, levels, info, t_end, t_start);
+        } catch (const std::exception& e) {
+            std::cerr << "Error writing trajectory: " << e.what() << '\n';
+            return 1;
+        }
+        output_s += lap(o0);
+    } else {
+        auto output = [&](int k) {
+            auto o0 = std::chrono::steady_clock::now();
+            check(s, npde_get_state(s, 0, U.data(), NPDE_LAYOUT_PLANES), bad);
+            try {
+                npde_io::write_snapshot(snap_path(k), g, U, t, info);
+            } catch (const std::exception& e) {
+                std::cerr << "Error writing snapshot: " << e.what() << '\n';
+                std::exit(1);
+            }
+            output_s += lap(o0);
+        };
+        double ms = 0.0;
+        check(s, npde_advance_to(s, t_start, &bad), bad);
+        npde_last_elapsed_ms(s, &ms);
+        device_ms += ms;
+        npde_get_time(s, 0, &t, &steps);
+        output(0);
+        for (int k = 1; k <= nout; ++k) {
+            const double t_next = t_start + (t_end - t_start) * k / nout;
+            check(s, npde_advance_to(s, t_next, &bad), bad);
+            npde_last_elapsed_ms(s, &ms);
+            device_ms += ms;
+            npde_get_time(s, 0, &t, &steps);
+            info.step = int(steps);
+            output(k);
+            std::printf("  snapshot %2d/%d  t = %.4f  step %lld\n", k, nout, t, steps);
+        }
+    }
+    const double secs = lap(t0);
+    const double mcells = double(g.nx) * g.ny * steps / secs / 1e6;
+    std::printf("Done: %lld steps in %.2f s (%.2f million cell-updates per second). Output in %s\n", steps, secs,
+                mcells, traj_mode ? a["traj"].c_str() : (dir + "/").c_str());
+    if (a["profile"] == "yes") {
+        const double cells = double(g.nx) * g.ny * double(steps > 0 ? steps : 1);
+        const double dev = device_ms * 1e-3, other = secs - dev - output_s;
+        std::printf("\n%-30s %9s %7s %12s\n", "phase", "time (s)", "share", "ns per cell");
+        const char* names[3] = {"device loop (2 stages + dt)", "output (copy + HDF5)", "host (setup, sync)"};
+        const double tt[3] = {dev, output_s, other > 0 ? other : 0.0};
+        for (int k = 0; k < 3; ++k)
+            std::printf("%-30s %9.3f %6.1f%% %12.2f\n", names[k], tt[k], 100.0 * tt[k] / secs, 1e9 * tt[k] / cells);
+        std::printf("%-30s %9.3f %6.1f%% %12.2f\n", "total", secs, 100.0, 1e9 * secs / cells);
+    }
+    return 0;
+}

@@ -66,6 +66,7 @@ _SIGNATURES = [
     ("npde_last_elapsed_ms", C.c_int, [_H, _D]),
     ("npde_stream", C.c_void_p, [_H]),
     ("npde_builtin_problem", C.c_int, [C.c_char_p, C.c_int, C.POINTER(GridDesc), _D, _D]),
+    ("npde_selftest_arith", C.c_int, [_D, _D, C.c_longlong, _D, _D, _D]),
 ]
 
 EXPORTED = [name for name, _, _ in _SIGNATURES]
