@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run a plain 1 GiB device copy (calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the batch would shard across GPUs "
@@ -80,6 +82,14 @@ def main():
     from npde.families import riemann
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 3 x 1 GiB device copies done")
+        del src, dst
     grids = [npde.make_grid(N, N, bc_x=npde.PERIODIC, bc_y=npde.PERIODIC)] * N_INST
     ics = [riemann(k, N)[0] for k in range(N_INST)]
     b = npde.Batch(grids, "hllc", "vanleer", 0.5)

@@ -3,6 +3,7 @@
 //  k_stage<SCHEME, LIM, STAGE, CORR>   one SSP-RK2 stage for every task of every stepping instance
 //  k_wavespeed                         max wave speeds + first non-physical cell of U (no update)
 //  k_frame                             block-averaged float32 frame of U (trajectory output)
+//  k_finish                            per-instance reduction of the partials and run-state decision
 //  k_prep_host_step                    host-supplied dt for the next step
 //
 // Stage kernel design (one wavefront = one task = 60 interior columns x a band of rows):
@@ -17,8 +18,8 @@
 //   * ghost cells are virtual: stencil indices outside the grid are remapped (periodic wrap or
 //     transmissive clamp) before the load;
 //   * stage 2 fuses the next step's dt reduction into its epilogue (to_primitive, positivity check,
-//     max(|u|+a), max(|v|+a) of U^{n+1}); the last wavefront of an instance to finish (device-scope
-//     counter) reduces the per-task partials and advances that instance's run state.
+//     max(|u|+a), max(|v|+a) of U^{n+1}) and writes one partial per task; k_finish (one wavefront per
+//     instance) reduces them and advances that instance's run state.
 //
 // All arithmetic is binary64 with the reference's operation order (euler_numerics.hpp); the file is
 // compiled with -ffp-contract=off.
@@ -70,27 +71,26 @@ __device__ __forceinline__ int wave_min(int x) {
     return x;
 }
 
-__device__ __forceinline__ Cons load4(const double* __restrict__ B, long long off, long long P) {
-    return {B[off], B[off + P], B[off + 2 * P], B[off + 3 * P]};
+// State pointers live in the instance descriptor, so the compiler cannot see that they address
+// global memory; casting to the global address space gives global_load/global_store (scalar base +
+// lane offset) instead of flat accesses.
+typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) double gdouble;
+
+__device__ __forceinline__ Cons load4(const double* B, long long off, long long P) {
+    const gcdouble* g = (const gcdouble*)B;
+    return {g[off], g[off + P], g[off + 2 * P], g[off + 3 * P]};
 }
 
-__device__ __forceinline__ void store4(double* __restrict__ B, long long off, long long P, const Cons& c) {
-    B[off] = c.d;
-    B[off + P] = c.m;
-    B[off + 2 * P] = c.n;
-    B[off + 3 * P] = c.e;
+__device__ __forceinline__ void store4(double* B, long long off, long long P, const Cons& c) {
+    gdouble* g = (gdouble*)B;
+    g[off] = c.d;
+    g[off + P] = c.m;
+    g[off + 2 * P] = c.n;
+    g[off + 3 * P] = c.e;
 }
 
 __device__ __forceinline__ Prim prim_of(const Cons& c) { return prim_of(c.d, c.m, c.n, c.e); }
-
-__device__ __forceinline__ Partial load_partial(const Partial* p) {
-    Partial r;
-    r.sx = __hip_atomic_load(&p->sx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    r.sy = __hip_atomic_load(&p->sy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    r.bad = __hip_atomic_load(&p->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    r.pad = 0;
-    return r;
-}
 
 // ---------------------------------------------------------------------------------------------
 // Run-state machine of one instance (executed by one lane).  Mirrors the driver loop
@@ -137,15 +137,16 @@ __device__ void decide_next(const InstDesc& D, Ctl* c, int* ndone) {
     }
 }
 
-// Called by every lane of the wavefront that completed an instance's last task of a launch:
-// reduce the per-task wave-speed partials, complete the step in flight, decide the next one.
+// Reduce an instance's per-task wave-speed partials (one wavefront), complete the step in flight and
+// decide the next one.  Runs in its own launch after the kernel that wrote the partials, so the
+// kernel boundary orders the partials before their reduction: no device-scope fence (which on this
+// chip writes back and invalidates a whole L2) is needed inside the stage kernels.
 __device__ void finish_instance(const InstDesc& D, Ctl* c, const Partial* __restrict__ part, int lane,
                                 int* ndone) {
-    __threadfence();
     double sx = 0.0, sy = 0.0;
     int bad = kNoBad;
     for (int t = lane; t < D.ntasks; t += kWave) {
-        const Partial p = load_partial(part + D.task0 + t);
+        const Partial p = part[D.task0 + t];
         sx = tmax(sx, p.sx);
         sy = tmax(sy, p.sy);
         bad = p.bad < bad ? p.bad : bad;
@@ -154,7 +155,6 @@ __device__ void finish_instance(const InstDesc& D, Ctl* c, const Partial* __rest
     sy = wave_max(sy);
     bad = wave_min(bad);
     if (lane == 0) {
-        atomicExch(&c->count, 0u);
         c->sx = sx;
         c->sy = sy;
         c->bad = bad;
@@ -167,22 +167,43 @@ __device__ void finish_instance(const InstDesc& D, Ctl* c, const Partial* __rest
     }
 }
 
-// Publish this task's partial and report whether it was the instance's last one.
-__device__ __forceinline__ bool publish_partial(Ctl* c, Partial* __restrict__ part, int tid, const InstDesc& D,
-                                                int lane, double sx, double sy, int bad) {
+// This task's maxima (all lanes) -> its partial slot.
+__device__ __forceinline__ void publish_partial(Partial* __restrict__ part, int tid, int lane, double sx,
+                                                double sy, int bad) {
     sx = wave_max(sx);
     sy = wave_max(sy);
     bad = wave_min(bad);
-    int last = 0;
     if (lane == 0) {
         part[tid].sx = sx;
         part[tid].sy = sy;
         part[tid].bad = bad;
-        __threadfence();
-        const unsigned prev = atomicAdd(&c->count, 1u);
-        last = (prev == unsigned(D.ntasks - 1)) ? 1 : 0;
     }
-    return __shfl(last, 0, kWave) != 0;
+}
+
+enum FinishMode : int { kAfterStep = 0, kAfterWavespeed = 1, kAfterFrame = 2 };
+
+// One wavefront per instance: after a stage-2 launch (instances that stepped), after a wave-speed
+// launch (instances not done), or after a frame launch (instances whose frame was just written).
+__global__ __launch_bounds__(64) void k_finish(const InstDesc* __restrict__ desc, Ctl* __restrict__ ctl,
+                                               const Partial* __restrict__ part, int n, int mode,
+                                               int* __restrict__ ndone) {
+    const int inst = blockIdx.x;
+    if (inst >= n) return;
+    Ctl* c = ctl + inst;
+    const InstDesc& D = desc[inst];
+    const int lane = threadIdx.x;
+    if (mode == kAfterFrame) {
+        if (c->pending < 0 || lane != 0) return;
+        const int k = c->pending;
+        if (D.ftimes) D.ftimes[k] = c->t;
+        c->pending = -1;
+        c->k = k + 1;
+        decide_next(D, c, ndone);
+        return;
+    }
+    const bool fresh = mode == kAfterStep ? c->active != 0 : c->done == 0;
+    if (!fresh) return;
+    finish_instance(D, c, part, lane, ndone);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -248,10 +269,16 @@ __global__ __launch_bounds__(256) void k_stage(const InstDesc* __restrict__ desc
     double sx = 0.0, sy = 0.0;
     int bad = kNoBad;
 
+    // software pipeline: the rows an iteration consumes were loaded one iteration earlier, so HBM
+    // latency overlaps a whole row of arithmetic instead of stalling the wavefront
+    Cons vn = load4(V, row(j0 + 2), P);
+    Cons ub{};
+    if constexpr (STAGE == 2) ub = load4(O, (long long)j0 * nx + ci, P);  // U, point-wise
+
     for (int f = j0; f < j1; ++f) {
-        const Cons vn = load4(V, row(f + 2), P);
-        Cons ub{};
-        if constexpr (STAGE == 2) ub = load4(O, (long long)f * nx + ci, P);  // U, point-wise
+        const Cons vq = load4(V, row(f + 3), P);
+        Cons uq{};
+        if constexpr (STAGE == 2) uq = load4(O, (long long)(f + 1 < ny ? f + 1 : f) * nx + ci, P);
         const Prim wn = prim_of(vn);
 
         // y: slope of row f+1, its two y-face sides, and the face between rows f and f+1
@@ -316,10 +343,12 @@ __global__ __launch_bounds__(256) void k_stage(const InstDesc* __restrict__ desc
         vb = vn;
         wa = wb;
         wb = wn;
+        vn = vq;
+        if constexpr (STAGE == 2) ub = uq;
     }
 
     if constexpr (STAGE == 2) {
-        if (publish_partial(c, part, tid, D, lane, sx, sy, bad)) finish_instance(D, c, part, lane, ndone);
+        publish_partial(part, tid, lane, sx, sy, bad);
     }
 }
 
@@ -341,8 +370,11 @@ __global__ __launch_bounds__(256) void k_wavespeed(const InstDesc* __restrict__ 
     const bool owner = lane >= kHalo && lane < kWave - kHalo && i < nx;
     double sx = 0.0, sy = 0.0;
     int bad = kNoBad;
+    const int ny = D.ny;
+    Cons un = load4(D.U, (long long)tk.j0 * nx + ci, P);  // one row in flight ahead of the arithmetic
     for (int f = tk.j0; f < tk.j1; ++f) {
-        const Cons u = load4(D.U, (long long)f * nx + ci, P);
+        const Cons u = un;
+        un = load4(D.U, (long long)(f + 1 < ny ? f + 1 : f) * nx + ci, P);
         const double rd = recip(u.d);
         const Prim w = prim_of(u.d, u.m, u.n, u.e, rd);
         const bool badc = !(w.d > 0.0) || !(w.p > 0.0);
@@ -357,7 +389,7 @@ __global__ __launch_bounds__(256) void k_wavespeed(const InstDesc* __restrict__ 
             }
         }
     }
-    if (publish_partial(c, part, tid, D, lane, sx, sy, bad)) finish_instance(D, c, part, lane, ndone);
+    publish_partial(part, tid, lane, sx, sy, bad);
 }
 
 // Block averages of the conserved variables for instances with a pending frame
@@ -396,19 +428,6 @@ __global__ __launch_bounds__(256) void k_frame(const InstDesc* __restrict__ desc
         out[pl + cell] = float(s1 * inv);
         out[2 * pl + cell] = float(s2 * inv);
         out[3 * pl + cell] = float(s3 * inv);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned prev = atomicAdd(&c->fcount, 1u);
-        if (prev == unsigned(D.nftasks - 1)) {
-            __threadfence();
-            atomicExch(&c->fcount, 0u);
-            if (D.ftimes) D.ftimes[k] = c->t;
-            c->pending = -1;
-            c->k = k + 1;
-            decide_next(D, c, ndone);
-        }
     }
 }
 
@@ -450,7 +469,8 @@ __global__ void k_prep_host_step(Ctl* __restrict__ ctl, int n, const double* __r
     ctl[k].active = dt >= 0.0 ? 1 : 0;
 }
 
-// Arithmetic self-test: the shared-reciprocal quotient, the plain IEEE quotient and sqrt, element-wise.
+// Arithmetic self-test, element-wise: the shared-reciprocal quotient and the plain IEEE quotient,
+// and the square root as the kernels evaluate it.
 __global__ void k_selftest_arith(const double* __restrict__ x, const double* __restrict__ y, long long n,
                                  double* __restrict__ q_fast, double* __restrict__ q_ieee,
                                  double* __restrict__ root) {
@@ -459,7 +479,7 @@ __global__ void k_selftest_arith(const double* __restrict__ x, const double* __r
     const double a = x[k], b = y[k];
     q_fast[k] = div_r(a, b, recip(b));
     q_ieee[k] = a / b;
-    root[k] = dsqrt(a);
+    root[k] = sqrt_core(a);
 }
 
 hipError_t launch_selftest_arith(const double* x, const double* y, long long n, double* qf, double* qi, double* rt,
@@ -515,6 +535,12 @@ hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int
                         hipStream_t s) {
     if (nftasks <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_frame, dim3(nftasks), dim3(256), 0, s, desc, ctl, ftasks, nftasks, ndone);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish(const InstDesc* desc, Ctl* ctl, const Partial* part, int n, int mode, int* ndone,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_finish, dim3(n), dim3(64), 0, s, desc, ctl, part, n, mode, ndone);
     return hipGetLastError();
 }
 

@@ -85,6 +85,29 @@ NPDE_HD double div_r(double x, double y, double) { return x / y; }
 // reciprocal of gamma - 1 for div_r(x, kGm1, rgm1()); a pure function, hoisted by the compiler
 NPDE_HD double rgm1() { return recip(kGm1); }
 
+// Square root: gfx950's correctly rounded binary64 sqrt is rsq(x) refined by a Goldschmidt step and two
+// FMA corrections, inside a pre/post scaling by 2^256/2^-128 for x < 2^-767 and a fix-up that returns
+// x itself for +-0 and +inf.  The sound speed's argument gamma*p/rho is never that small for a state
+// with p > 0, so the core runs without the scaling (the fix-up is kept); for x >= 2^-767, zeros,
+// infinities, negatives and NaNs this is the compiler's sequence, hence the IEEE result.
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(NPDE_IEEE_DIV)
+NPDE_HD double sqrt_core(double x) {
+    const double g = __builtin_amdgcn_rsq(x);
+    double s = x * g;
+    double h = g * 0.5;
+    const double r = __builtin_fma(-h, s, 0.5);
+    s = __builtin_fma(s, r, s);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-s, s, x);
+    s = __builtin_fma(d, h, s);
+    d = __builtin_fma(-s, s, x);
+    s = __builtin_fma(d, h, s);
+    return __builtin_amdgcn_class(x, 0x260 /* -0, +0, +inf */) ? x : s;
+}
+#else
+NPDE_HD double sqrt_core(double x) { return dsqrt(x); }
+#endif
+
 NPDE_HD Prim prim_of(double d, double m, double n, double e, double rd) {
     const double u = div_r(m, d, rd);
     const double v = div_r(n, d, rd);
@@ -99,7 +122,7 @@ NPDE_HD Cons cons_of(const Prim& w) {
     return {w.d, w.d * w.u, w.d * w.v, div_r(w.p, kGm1, rgm1()) + kin};
 }
 
-NPDE_HD double sound(double d, double p, double rd) { return dsqrt(div_r(kGamma * p, d, rd)); }
+NPDE_HD double sound(double d, double p, double rd) { return sqrt_core(div_r(kGamma * p, d, rd)); }
 NPDE_HD double sound(double d, double p) { return sound(d, p, recip(d)); }
 
 template <int LIM>

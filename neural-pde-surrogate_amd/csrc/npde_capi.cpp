@@ -29,6 +29,9 @@ hipError_t launch_wavespeed(int waves_per_block, const InstDesc* desc, Ctl* ctl,
 hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int nftasks, int* ndone,
                         hipStream_t s);
 hipError_t launch_block_average(const InstDesc* desc, int inst, int m, int nyc, float* out, hipStream_t s);
+hipError_t launch_finish(const InstDesc* desc, Ctl* ctl, const Partial* part, int n, int mode, int* ndone,
+                         hipStream_t s);
+enum : int { kAfterStep = 0, kAfterWavespeed = 1, kAfterFrame = 2 };
 hipError_t launch_prep_host_step(Ctl* ctl, int n, const double* dts, hipStream_t s);
 hipError_t launch_selftest_arith(const double* x, const double* y, long long n, double* qf, double* qi, double* rt,
                                  hipStream_t s);
@@ -166,17 +169,30 @@ npde_status fill_bad(npde_solver* s, npde_bad_cell* bad) {
     return NPDE_OK;
 }
 
+// One RK2 step of every stepping instance: two stage launches, then the per-instance reduction of
+// the fused wave-speed partials and the run-state decision.
 npde_status launch_step_pair(npde_solver* s) {
     HIPCHK(s, launch_stage(s->flux, s->lim, 1, s->has_corr, s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks,
                            int(s->htasks.size()), s->dpart, s->dndone, s->stream));
     HIPCHK(s, launch_stage(s->flux, s->lim, 2, s->has_corr, s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks,
                            int(s->htasks.size()), s->dpart, s->dndone, s->stream));
+    HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterStep, s->dndone, s->stream));
+    return NPDE_OK;
+}
+
+npde_status launch_wavespeed_and_finish(npde_solver* s) {
+    HIPCHK(s, launch_wavespeed(s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks, int(s->htasks.size()),
+                               s->dpart, s->dndone, s->stream));
+    HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterWavespeed, s->dndone, s->stream));
     return NPDE_OK;
 }
 
 npde_status launch_iterations(npde_solver* s, int iters, int nft) {
     for (int it = 0; it < iters; ++it) {
-        if (nft > 0) HIPCHK(s, launch_frame(s->ddesc, s->dctl, s->dftasks, nft, s->dndone, s->stream));
+        if (nft > 0) {
+            HIPCHK(s, launch_frame(s->ddesc, s->dctl, s->dftasks, nft, s->dndone, s->stream));
+            HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterFrame, s->dndone, s->stream));
+        }
         npde_status st = launch_step_pair(s);
         if (st != NPDE_OK) return st;
     }
@@ -229,8 +245,6 @@ npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, boo
         c.done = 0;
         c.error = 0;
         c.active = 0;
-        c.count = 0;
-        c.fcount = 0;
     }
     if ((st = push_ctl(s)) != NPDE_OK) return st;
     HIPCHK(s, hipMemsetAsync(s->dndone, 0, sizeof(int), s->stream));
@@ -239,8 +253,7 @@ npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, boo
         if ((st = ensure_graph(s, nft)) != NPDE_OK) return st;
     }
     HIPCHK(s, hipEventRecord(s->ev0, s->stream));
-    HIPCHK(s, launch_wavespeed(s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks, int(s->htasks.size()),
-                               s->dpart, s->dndone, s->stream));
+    if ((st = launch_wavespeed_and_finish(s)) != NPDE_OK) return st;
     int done = 0;
     const int K = std::max(1, s->tune.steps_per_graph);
     // A state whose wave speed is infinite gives dt = 0 and would never reach its target (the reference
@@ -275,8 +288,6 @@ npde_status set_host_mode(npde_solver* s) {
         c.error = 0;
         c.active = 0;
         c.pending = -1;
-        c.count = 0;
-        c.fcount = 0;
     }
     return push_ctl(s);
 }
@@ -493,8 +504,7 @@ npde_status npde_compute_dt(npde_solver* s, double* dt, npde_bad_cell* bad) {
     if (!s || !dt) return fail(s, NPDE_ERR_INVALID, "null argument");
     npde_status st = set_host_mode(s);
     if (st != NPDE_OK) return st;
-    HIPCHK(s, launch_wavespeed(s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks, int(s->htasks.size()),
-                               s->dpart, s->dndone, s->stream));
+    if ((st = launch_wavespeed_and_finish(s)) != NPDE_OK) return st;
     if ((st = pull_ctl(s)) != NPDE_OK) return st;
     for (int k = 0; k < s->n; ++k) {
         const Ctl& c = s->hctl[k];
@@ -549,6 +559,7 @@ npde_status npde_time_stages(npde_solver* s, int nsteps, double* ms_stage1, doub
         HIPCHK(s, launch_stage(s->flux, s->lim, 2, s->has_corr, s->tune.waves_per_block, s->ddesc, s->dctl,
                                s->dtasks, int(s->htasks.size()), s->dpart, s->dndone, s->stream));
         HIPCHK(s, hipEventRecord(e[2], s->stream));
+        HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterStep, s->dndone, s->stream));
         HIPCHK(s, hipEventSynchronize(e[2]));
         float a = 0.f, b = 0.f;
         HIPCHK(s, hipEventElapsedTime(&a, e[0], e[1]));

