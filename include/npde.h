@@ -105,13 +105,31 @@ npde_status npde_block_average(npde_solver* s, int instance, int m, float* out);
 /* ---- learned flux correction (coarse solver) ---- */
 /* Device pointers (4 x ny x nx binary64 each) holding per-face increments added to the face fluxes
  * of every subsequent stage: corr_x[q][j][i] on the face left of cell (i, j), corr_y[q][j][i] on the
- * face below it.  Pass NULLs to switch the correction off. */
+ * face below it (on a transmissive boundary the outermost faces get none).  Pass NULLs to switch the
+ * correction off for that instance.  The pointers are read by the kernels until changed: the caller
+ * keeps the buffers alive and finishes writing them before the next step.  nx*ny must be a multiple
+ * of 64.  No counterpart in the reference. */
 npde_status npde_set_flux_correction(npde_solver* s, int instance, const double* corr_x, const double* corr_y);
 
 /* ---- built-in problems of the driver: sod_x, sod_y, lax_liu3, sine2d, kh (main2d.cpp:52-135) ---- */
 /* Fills grid and t_end; if U is non-NULL also the initial state as 4 x ny x nx planes.  Call once
  * with U = NULL to learn the grid size. */
 npde_status npde_builtin_problem(const char* test, int n, npde_grid_desc* grid, double* t_end, double* U);
+
+/* ---- learned-correction network kernels (no counterpart in the reference: its networks are JAX;
+ *      closest analog the periodic 3x3 convolution of surrogate/models.py:16-29) ----
+ * Device pointers; activations NHWC float32 with channels a multiple of 16, values rounded to bfloat16
+ * at the MFMA inputs, FP32 accumulation.  wpack: bfloat16 weights in MFMA operand order
+ * [tap][cin/16][cout/16][64][4].  epilogue: 0 linear (+bias), 1 +bias then tanh-GELU (pre-activation
+ * to `pre` if non-NULL), 2 multiply by GELU'(gpre) (data gradient).  Return 0 on success. */
+int npde_conv3x3(const float* x, int B, int H, int W, int cin, const void* wpack, const float* bias, int cout,
+                 int cout_store, float* y, float* pre, int epilogue, const float* gpre, void* stream);
+/* Weight/bias gradient of a 3x3 periodic convolution: dw[tap][ci][co], db[co], summed over bands of
+ * `rows` image rows in a fixed order (part, bpart: scratch of (B*H/rows)*9*cin*cout and (B*H/rows)*cout). */
+int npde_conv3x3_wgrad(const float* x, const float* g, int B, int H, int W, int cin, int cout, int rows,
+                       float* part, float* bpart, float* dw, float* db, void* stream);
+/* One 16x16x16 BF16 MFMA on row-major 16x16 float inputs (operand-layout check). */
+int npde_mfma_probe(const float* a, const float* b, float* d, void* stream);
 
 /* ---- diagnostics (no counterpart in the reference) ---- */
 /* Element-wise on the device: q_fast = x / y by the kernels' shared-reciprocal sequence, q_ieee = x / y

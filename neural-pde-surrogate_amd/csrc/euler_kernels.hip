@@ -234,14 +234,17 @@ __global__ __launch_bounds__(256) void k_stage(const InstDesc* __restrict__ desc
 
     // learned flux increments (coarse solver only): face left of column i, face below row r
     int cxi = -1;
+    // an instance of a corrected batch without increments of its own reads none (null pointers)
+    const bool has_corr = CORR && D.corr_x != nullptr && D.corr_y != nullptr;
     if constexpr (CORR) cxi = (i >= 0 && i < nx) ? i : (D.bc_x == kPeriodic ? remap(i, nx, kPeriodic) : -1);
+    if (!has_corr) cxi = -1;
     auto corr_x = [&](int r) -> Cons {
         if (cxi < 0) return {0.0, 0.0, 0.0, 0.0};
         return load4(D.corr_x, (long long)r * nx + cxi, P);
     };
     auto corr_y = [&](int r) -> Cons {  // r in [0, ny]
         int rr = r < ny ? r : (D.bc_y == kPeriodic ? 0 : -1);
-        if (rr < 0) return {0.0, 0.0, 0.0, 0.0};
+        if (rr < 0 || !has_corr) return {0.0, 0.0, 0.0, 0.0};
         return load4(D.corr_y, (long long)rr * nx + ci, P);
     };
 

@@ -686,12 +686,9 @@ npde_status npde_set_flux_correction(npde_solver* s, int inst, const double* cor
     D.corr_y = corr_y;
     bool any = false;
     for (const auto& d : s->hdesc) any = any || d.corr_x != nullptr;
-    if (any) {
-        for (const auto& d : s->hdesc)
-            if (!d.corr_x) return fail(s, NPDE_ERR_STATE, "set a correction on every instance of the batch (or none)");
-        for (const auto& d : s->hdesc)
-            if (d.plane != (long long)(size_t(d.nx) * d.ny))
-                return fail(s, NPDE_ERR_INVALID, "flux correction needs nx*ny to be a multiple of 64");
+    if (corr_x && D.plane != (long long)(size_t(D.nx) * D.ny)) {
+        D.corr_x = D.corr_y = nullptr;
+        return fail(s, NPDE_ERR_INVALID, "flux correction needs nx*ny to be a multiple of 64");
     }
     s->has_corr = any;
     HIPCHK(s, hipMemcpy(s->ddesc, s->hdesc.data(), sizeof(InstDesc) * s->n, hipMemcpyHostToDevice));

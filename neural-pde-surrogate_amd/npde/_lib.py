@@ -67,6 +67,11 @@ _SIGNATURES = [
     ("npde_stream", C.c_void_p, [_H]),
     ("npde_builtin_problem", C.c_int, [C.c_char_p, C.c_int, C.POINTER(GridDesc), _D, _D]),
     ("npde_selftest_arith", C.c_int, [_D, _D, C.c_longlong, _D, _D, _D]),
+    ("npde_conv3x3", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                               C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    ("npde_conv3x3_wgrad", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("npde_mfma_probe", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
 ]
 
 EXPORTED = [name for name, _, _ in _SIGNATURES]

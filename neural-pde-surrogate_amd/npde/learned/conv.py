@@ -1,0 +1,135 @@
+"""The flux-correction network's forward and backward on the project's BF16-MFMA kernels.
+
+FluxCNN.apply(features, w1, b1, ..., wL, bL) is one torch.autograd.Function for the whole stack
+  z_1 = conv(x) + b_1,  h_k = GELU(z_k),  z_{k+1} = conv(h_k) + b_{k+1},  out = z_L
+(3x3 periodic convolutions, tanh-approximate GELU on hidden layers, linear head).  Its forward runs
+npde_conv3x3 once per layer (bias + GELU fused in the epilogue, pre-activations kept for backward);
+its backward runs, per layer from the top, npde_conv3x3_wgrad (weight and bias gradients) and the
+data-gradient convolution with GELU'(z_{k-1}) fused into its epilogue.
+
+Tensors: activations NHWC float32 on the GPU with channels padded to a multiple of 16 (values enter
+the MFMAs rounded to bfloat16); weights (Cout, Cin, 3, 3) float32 master copies, packed to BF16
+MFMA operand order on every call.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from .. import _lib
+
+EPI_LINEAR, EPI_GELU, EPI_GELU_GRAD = 0, 1, 2
+WGRAD_ROWS = 8
+
+
+def _pad16(c: int) -> int:
+    return (c + 15) // 16 * 16
+
+
+def pack_weights(w: torch.Tensor, transpose_flip: bool = False) -> torch.Tensor:
+    """(Cout, Cin, 3, 3) float32 -> bfloat16 [tap][ci/16][co/16][lane][4] with B[k][n] = W[ci][co] where
+    lane = 4*(k/4)... i.e. element (lane = kq*16 + m, i) holds ci = c16*16 + 4*kq + i, co = cot*16 + m.
+    transpose_flip packs the data-gradient operator (taps reversed, ci and co exchanged)."""
+    cout, cin = w.shape[:2]
+    ci_p, co_p = _pad16(cin), _pad16(cout)
+    wp = w.new_zeros(co_p, ci_p, 3, 3)
+    wp[:cout, :cin] = w
+    wt = wp.permute(2, 3, 1, 0).reshape(9, ci_p, co_p)  # [tap = ky*3 + kx][ci][co]
+    if transpose_flip:
+        wt = wt.flip(0).transpose(1, 2)
+    n_in, n_out = wt.shape[1], wt.shape[2]
+    wt = wt.reshape(9, n_in // 16, 4, 4, n_out // 16, 16).permute(0, 1, 4, 2, 5, 3)
+    return wt.contiguous().to(torch.bfloat16).contiguous()
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with status {rc}")
+
+
+def conv3x3(x, wpack, bias, cout_p, cout_store, epilogue, want_pre=False, gpre=None):
+    lib = _lib.load()
+    B, H, W, cin = x.shape
+    y = torch.empty(B, H, W, cout_store, device=x.device, dtype=torch.float32)
+    pre = torch.empty(B, H, W, cout_p, device=x.device, dtype=torch.float32) if want_pre else None
+    _check(lib.npde_conv3x3(_ptr(x), B, H, W, cin, _ptr(wpack), _ptr(bias), cout_p, cout_store, _ptr(y), _ptr(pre),
+                            epilogue, _ptr(gpre), _stream()), "npde_conv3x3")
+    return y, pre
+
+
+def conv3x3_wgrad(x, g):
+    """x: (B,H,W,Cin_p) layer input, g: (B,H,W,Cout_p) gradient of its pre-activation ->
+    dW (Cout_p, Cin_p, 3, 3), db (Cout_p,)."""
+    lib = _lib.load()
+    B, H, W, cin = x.shape
+    cout = g.shape[-1]
+    nb = B * (H // WGRAD_ROWS)
+    part = torch.empty(nb * 9 * cin * cout, device=x.device, dtype=torch.float32)
+    bpart = torch.empty(nb * cout, device=x.device, dtype=torch.float32)
+    dw = torch.empty(9, cin, cout, device=x.device, dtype=torch.float32)
+    db = torch.empty(cout, device=x.device, dtype=torch.float32)
+    _check(lib.npde_conv3x3_wgrad(_ptr(x), _ptr(g), B, H, W, cin, cout, WGRAD_ROWS, _ptr(part), _ptr(bpart), _ptr(dw),
+                                  _ptr(db), _stream()), "npde_conv3x3_wgrad")
+    return dw.view(3, 3, cin, cout).permute(3, 2, 0, 1), db
+
+
+class FluxCNN(torch.autograd.Function):
+    """Whole network as one autograd node; see the module docstring."""
+
+    @staticmethod
+    def forward(ctx, x, *params):
+        ws, bs = params[0::2], params[1::2]
+        L = len(ws)
+        hs, zs = [x], []
+        h = x
+        for k in range(L):
+            cout = ws[k].shape[0]
+            cout_p = _pad16(cout)
+            bias = torch.zeros(cout_p, device=x.device, dtype=torch.float32)
+            bias[:cout] = bs[k]
+            last = k == L - 1
+            y, z = conv3x3(h, pack_weights(ws[k]), bias, cout_p, cout if last else cout_p,
+                           EPI_LINEAR if last else EPI_GELU, want_pre=not last)
+            if not last:
+                zs.append(z)
+                hs.append(y)
+            h = y
+        ctx.save_for_backward(*hs, *zs, *ws)
+        ctx.L = L
+        return h
+
+    @staticmethod
+    def backward(ctx, gout):
+        L = ctx.L
+        saved = ctx.saved_tensors
+        hs, zs, ws = saved[:L], saved[L:2 * L - 1], saved[2 * L - 1:]
+        grads_w, grads_b = [None] * L, [None] * L
+        cout = ws[-1].shape[0]
+        g = torch.zeros(*gout.shape[:-1], _pad16(cout), device=gout.device, dtype=torch.float32)
+        g[..., :cout] = gout
+        gx = None
+        for k in range(L - 1, -1, -1):
+            co, ci = ws[k].shape[:2]
+            dw, db = conv3x3_wgrad(hs[k], g)
+            grads_w[k] = dw[:co, :ci].contiguous()
+            grads_b[k] = db[:co].contiguous()
+            if k > 0 or ctx.needs_input_grad[0]:
+                cin_p = hs[k].shape[-1]
+                epi = EPI_GELU_GRAD if k > 0 else EPI_LINEAR
+                g, _ = conv3x3(g, pack_weights(ws[k], transpose_flip=True), None, cin_p, cin_p, epi,
+                               gpre=zs[k - 1] if k > 0 else None)
+                if k == 0:
+                    gx = g
+        out = [gx]
+        for k in range(L):
+            out += [grads_w[k], grads_b[k]]
+        return tuple(out)

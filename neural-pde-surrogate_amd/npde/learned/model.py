@@ -1,0 +1,83 @@
+"""Conservative learned flux correction for the coarse solver.
+
+A stack of 3x3 periodic convolutions (4 hidden layers of 32 channels with tanh-GELU, then a linear
+head; 32,392 parameters) maps the coarse state to 8 per-face flux increments: 4 conserved components
+on the face left of each cell and 4 on the face below it.  The increments are added to the solver's
+face fluxes (HLLC + van Leer) inside the stage kernels, so the update stays in flux form: mass,
+momentum and energy are conserved to rounding whatever the network outputs.  The head is zero
+initialised, so an untrained corrector reproduces the plain solver exactly (analog: the zero-init
+last layer of the reference's surrogates, surrogate/models.py:66-69).
+
+Inputs are 8 features per cell (conserved and primitive variables and sound speed, shifted and
+scaled to order one) padded to 16 channels for the MFMA kernels.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+from .conv import FluxCNN
+
+GAMMA = 1.4
+IN_FEATURES = 8
+# (offset, scale) per feature: rho, rho*u, rho*v, E, u, v, p, a
+FEATURE_NORM = [(1.0, 1.0), (0.0, 1.0), (0.0, 1.0), (2.5, 2.5), (0.0, 1.0), (0.0, 1.0), (1.0, 1.0), (1.2, 0.6)]
+
+
+def features(U: torch.Tensor) -> torch.Tensor:
+    """(B, 4, H, W) float64 conserved -> (B, H, W, 16) float32 normalised features (channels 8-15 zero)."""
+    d, m, n, e = U[:, 0], U[:, 1], U[:, 2], U[:, 3]
+    u, v = m / d, n / d
+    p = (GAMMA - 1.0) * (e - 0.5 * d * (u * u + v * v))
+    a = torch.sqrt(torch.clamp(GAMMA * p / d, min=0.0))
+    f = torch.stack([d, m, n, e, u, v, p, a], dim=-1)
+    off = torch.tensor([o for o, _ in FEATURE_NORM], dtype=f.dtype, device=f.device)
+    sc = torch.tensor([s for _, s in FEATURE_NORM], dtype=f.dtype, device=f.device)
+    f = ((f - off) / sc).to(torch.float32)
+    return torch.cat([f, torch.zeros_like(f)], dim=-1).contiguous()
+
+
+class FluxCorrector(torch.nn.Module):
+    def __init__(self, hidden: int = 32, hidden_layers: int = 4, out_scale: float = 0.05, seed: int = 0):
+        super().__init__()
+        chans = [IN_FEATURES] + [hidden] * hidden_layers + [8]
+        g = torch.Generator().manual_seed(seed)
+        self.weights = torch.nn.ParameterList()
+        self.biases = torch.nn.ParameterList()
+        for k in range(len(chans) - 1):
+            cin, cout = chans[k], chans[k + 1]
+            last = k == len(chans) - 2
+            std = 0.0 if last else math.sqrt(2.0 / (9 * cin))
+            w = torch.randn(cout, cin, 3, 3, generator=g) * std
+            self.weights.append(torch.nn.Parameter(w))
+            self.biases.append(torch.nn.Parameter(torch.zeros(cout)))
+        self.out_scale = out_scale
+
+    def params(self):
+        out = []
+        for w, b in zip(self.weights, self.biases):
+            out += [w, b]
+        return out
+
+    def num_parameters(self) -> int:
+        return sum(p.numel() for p in self.parameters())
+
+    def forward(self, U: torch.Tensor):
+        """(B, 4, H, W) float64 state -> (corr_x, corr_y), each (B, 4, H, W) float64."""
+        out = FluxCNN.apply(features(U), *self.params())  # (B, H, W, 8) float32
+        out = (out * self.out_scale).to(torch.float64)
+        cx = out[..., 0:4].permute(0, 3, 1, 2).contiguous()
+        cy = out[..., 4:8].permute(0, 3, 1, 2).contiguous()
+        return cx, cy
+
+    def state(self) -> dict:
+        return {f"w{k}": w.detach().cpu() for k, w in enumerate(self.weights)} | \
+               {f"b{k}": b.detach().cpu() for k, b in enumerate(self.biases)} | \
+               {"out_scale": torch.tensor(self.out_scale)}
+
+    def load(self, st: dict):
+        for k in range(len(self.weights)):
+            self.weights[k].data.copy_(torch.as_tensor(st[f"w{k}"]))
+            self.biases[k].data.copy_(torch.as_tensor(st[f"b{k}"]))
+        self.out_scale = float(st["out_scale"])

@@ -1,0 +1,293 @@
+"""Data generation, training and evaluation of the learned flux correction (random-Riemann family).
+
+  python -m npde.learned.pipeline --out DIR [--train-traj 448] [--iters 600] ...
+
+1. Truth: the random-Riemann generator (npde.families.riemann, indices 0..N-1) solved at 512^2 on the
+   GPU in batches of 64 (HLLC + van Leer, CFL 0.5, periodic, t_end 0.5), frames every 0.01 block-
+   averaged to 64^2 (float32), as the reference's dataset pipeline does with separate processes
+   (data/generate.py:60-105; trajectory.hpp:31-51).
+2. Training: batch 32 of (trajectory, start frame); the coarse 64^2 solver with the network's flux
+   increments is unrolled over 8 frame intervals (each interval: one network evaluation, then CFL
+   substeps landing on the frame time), loss = mean over intervals of the normalised L1 error of the
+   conserved variables; the backward recomputes each interval's forward (activation
+   checkpointing); AdamW with warm-up + cosine decay and global-norm clipping 1.0 (the reference's
+   optimiser chain, surrogate/train.py:209-211).
+3. Evaluation: held-out indices; density L1 against the block-averaged 512^2 truth at every frame,
+   averaged over frames (the protocol of experiments/solver_baseline.py:96-124), for the plain GPU
+   solver at 64, 128 and 256 and the learned solver at 64; wall times of each batch run; the plain
+   resolution matching the learned error and the matched-accuracy speed-up, by log-log interpolation.
+
+Fallback: after every interval a learned instance whose density or pressure left (0, bound] (bound =
+6 x the largest initial density / pressure: an ideal-gas shock compresses at most 6x at gamma = 1.4)
+or became non-finite is re-solved by the plain binary64 fine solve, whose block averages replace it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import time
+from pathlib import Path
+
+import numpy as np
+import torch
+import torch.utils.checkpoint
+
+from .. import solver as S
+from ..families import riemann
+from . import torch_solver as ts
+from .model import FluxCorrector
+
+FINE, COARSE = 512, 64
+T_END, NOUT = 0.5, 50
+
+
+def frame_times(nout=NOUT, t_end=T_END):
+    return [0.0 + (t_end - 0.0) * k / nout for k in range(nout + 1)]
+
+
+def block_average_f64(U: np.ndarray, m: int) -> np.ndarray:
+    """(4, n, n) -> (4, m, m) binary64 block means (row sums then column sums)."""
+    f = U.shape[-1] // m
+    return U.reshape(4, m, f, m, f).sum(axis=(2, 4)) / (f * f)
+
+
+def generate(indices, n_fine=FINE, sizes=(COARSE,), batch=64, log=print):
+    """Solve the generator's instances at n_fine; returns {m: float32 (N, NOUT+1, 4, m, m)} and seconds."""
+    out = {m: [] for m in sizes}
+    t0 = time.perf_counter()
+    grids = None
+    for s0 in range(0, len(indices), batch):
+        idx = list(indices[s0:s0 + batch])
+        if grids is None or len(grids) != len(idx):
+            grids = [S.make_grid(n_fine, n_fine, bc_x=S.PERIODIC, bc_y=S.PERIODIC)] * len(idx)
+        with S.Batch(grids, "hllc", "vanleer", 0.5) as b:
+            for k, i in enumerate(idx):
+                b.set_state(k, riemann(i, n_fine)[0])
+            b.run_trajectory(T_END, NOUT, sizes=sizes)
+            for m in sizes:
+                out[m].append(np.stack([b.trajectory(k, m) for k in range(len(idx))]))
+        log(f"  generated {s0 + len(idx)}/{len(indices)} trajectories at {n_fine}^2 "
+            f"({time.perf_counter() - t0:.1f} s)")
+    return {m: np.concatenate(v) for m, v in out.items()}, time.perf_counter() - t0
+
+
+SCALES = torch.tensor([1.0, 1.0, 1.0, 2.5], dtype=torch.float64)
+
+
+def train(data: torch.Tensor, iters=600, batch=32, unroll=8, lr=2e-3, warmup=50, seed=0, log=print,
+          model: FluxCorrector | None = None):
+    """data: (N, NOUT+1, 4, 64, 64) float32 on the GPU."""
+    dev = data.device
+    model = model or FluxCorrector(seed=seed)
+    model = model.to(dev)
+    opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=1e-4)
+    sched = torch.optim.lr_scheduler.LambdaLR(
+        opt, lambda i: min(1.0, (i + 1) / warmup) * 0.5 * (1 + math.cos(math.pi * min(1.0, i / iters))))
+    gen = torch.Generator().manual_seed(seed)
+    N, F = data.shape[0], data.shape[1]
+    n = data.shape[-1]
+    dx = 1.0 / n
+    tf = torch.tensor(frame_times(F - 1), dtype=torch.float64, device=dev)
+    scales = SCALES.to(dev).view(1, 4, 1, 1)
+    hist = []
+    t0 = time.perf_counter()
+    for it in range(iters):
+        traj = torch.randint(0, N, (batch,), generator=gen)
+        k0 = torch.randint(0, F - 1 - unroll + 1, (batch,), generator=gen)
+        U = data[traj, k0].to(torch.float64)
+        t = tf[k0.to(dev)]
+        loss = 0.0
+        for j in range(1, unroll + 1):
+            tn = tf[(k0 + j).to(dev)]
+
+            def interval(U_, t_, tn_=tn):
+                cx, cy = model(U_)
+                U2, t2, _ = ts.advance(U_, t_, tn_, dx, dx, cx_corr=cx, cy_corr=cy)
+                return U2, t2
+
+            U, t = torch.utils.checkpoint.checkpoint(interval, U, t, use_reentrant=False)
+            target = data[traj, k0 + j].to(torch.float64)
+            loss = loss + ((U - target).abs() / scales).mean()
+        loss = loss / unroll
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        gnorm = torch.nn.utils.clip_grad_norm_(model.parameters(), 1.0)
+        opt.step()
+        sched.step()
+        hist.append(float(loss))
+        if it % 25 == 0 or it == iters - 1:
+            log(f"  iter {it:4d}  loss {float(loss):.5f}  |g| {float(gnorm):.3g}  lr {sched.get_last_lr()[0]:.2e}  "
+                f"({time.perf_counter() - t0:.0f} s)")
+    return model, hist
+
+
+def l1_density(pred: np.ndarray, truth: np.ndarray) -> np.ndarray:
+    """(N, F, 4, m, m) pair -> (N, F) cell-averaged |rho_pred - rho_true| (area-weighted L1 on [0,1]^2)."""
+    return np.abs(pred[:, :, 0].astype(np.float64) - truth[:, :, 0].astype(np.float64)).mean(axis=(-1, -2))
+
+
+def plain_rollout(indices, n, m=COARSE, batch=64):
+    """Plain GPU solver at n from the block-averaged fine IC; frames block-averaged to m. Returns frames
+    (N, F, 4, m, m) float32 and the device-loop seconds (sum over batches)."""
+    frames, secs = [], 0.0
+    for s0 in range(0, len(indices), batch):
+        idx = list(indices[s0:s0 + batch])
+        grids = [S.make_grid(n, n, bc_x=S.PERIODIC, bc_y=S.PERIODIC)] * len(idx)
+        with S.Batch(grids, "hllc", "vanleer", 0.5) as b:
+            for k, i in enumerate(idx):
+                b.set_state(k, block_average_f64(riemann(i, FINE)[0], n))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.run_trajectory(T_END, NOUT, sizes=(m,))
+            secs += time.perf_counter() - t0
+            frames.append(np.stack([b.trajectory(k, m) for k in range(len(idx))]))
+    return np.concatenate(frames), secs
+
+
+def learned_rollout(model: FluxCorrector, indices, n=COARSE, batch=64, force_fallback=False, log=print):
+    """Coarse HIP solver with the network's increments; frames (N, F, 4, n, n) float32, seconds,
+    and the indices that fell back to the plain fine solve (whose frames replace theirs)."""
+    dev = torch.device("cuda")
+    tf = frame_times()
+    frames, secs, fell = [], 0.0, []
+    for s0 in range(0, len(indices), batch):
+        idx = list(indices[s0:s0 + batch])
+        B = len(idx)
+        U0 = np.stack([block_average_f64(riemann(i, FINE)[0], n) for i in idx])
+        rho_max = U0[:, 0].max(axis=(-1, -2))
+        p0 = (1.4 - 1.0) * (U0[:, 3] - 0.5 * (U0[:, 1] ** 2 + U0[:, 2] ** 2) / U0[:, 0])
+        p_max = p0.max(axis=(-1, -2))
+        grids = [S.make_grid(n, n, bc_x=S.PERIODIC, bc_y=S.PERIODIC)] * B
+        out = np.zeros((B, NOUT + 1, 4, n, n), dtype=np.float32)
+        out[:, 0] = U0.astype(np.float32)
+        bad = np.zeros(B, dtype=bool)
+        with S.Batch(grids, "hllc", "vanleer", 0.5) as b, torch.no_grad():
+            b.set_tuning(steps_per_graph=8)  # ~5 substeps per frame interval at 64^2
+            U = torch.from_numpy(U0).to(dev)
+            CX = torch.zeros_like(U)  # increments live in fixed buffers the kernels read every step
+            CY = torch.zeros_like(U)
+            for k in range(B):
+                b.set_state(k, U0[k])
+                b.set_flux_correction(k, CX[k].data_ptr(), CY[k].data_ptr())
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for j in range(1, NOUT + 1):
+                cx, cy = model(U)
+                keep = torch.from_numpy(~bad).to(dev).view(-1, 1, 1, 1)
+                CX.copy_(torch.where(keep, cx, torch.zeros_like(cx)))
+                CY.copy_(torch.where(keep, cy, torch.zeros_like(cy)))
+                torch.cuda.synchronize()
+                try:
+                    b.advance_to(tf[j])
+                except S.NonPhysicalStateError as e:
+                    bad[e.instance] = True  # stop trusting it; its later frames come from the fine solve
+                for k in range(B):
+                    b.get_state_device(k, U[k].data_ptr())
+                rho, p = U[:, 0], (1.4 - 1.0) * (U[:, 3] - 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2) / U[:, 0])
+                ok = (torch.isfinite(U).flatten(1).all(1) & (rho.flatten(1).amin(1) > 0) & (p.flatten(1).amin(1) > 0)
+                      & (rho.flatten(1).amax(1) <= 6 * torch.from_numpy(rho_max).to(dev))
+                      & (p.flatten(1).amax(1) <= 6 * torch.from_numpy(p_max).to(dev)))
+                bad |= ~ok.cpu().numpy()
+                if force_fallback:
+                    bad[:] = True
+                out[:, j] = U.to(torch.float32).cpu().numpy()
+            torch.cuda.synchronize()
+            secs += time.perf_counter() - t0
+        if bad.any():
+            redo = [idx[k] for k in np.nonzero(bad)[0]]
+            t1 = time.perf_counter()
+            fine, _ = generate(redo, sizes=(n,), log=lambda *a: None)
+            secs += time.perf_counter() - t1
+            for r, k in enumerate(np.nonzero(bad)[0]):
+                out[k] = fine[n][r]
+            fell += redo
+            log(f"  fallback to the fine solve for {len(redo)} instance(s): {redo[:8]}")
+        frames.append(out)
+    return np.concatenate(frames), secs, fell
+
+
+def matched_speedup(ns, errs, times, e_target, t_learned):
+    """Plain resolution whose error equals e_target (log-log interpolation in n) and its time."""
+    ln, le, lt = np.log(ns), np.log(errs), np.log(times)
+    if e_target >= errs[0]:
+        return float(ns[0]), float(times[0]), float(times[0] / t_learned)
+    for a in range(len(ns) - 1):
+        if errs[a] >= e_target >= errs[a + 1]:
+            w = (math.log(e_target) - le[a]) / (le[a + 1] - le[a])
+            n_star = math.exp(ln[a] + w * (ln[a + 1] - ln[a]))
+            t_star = math.exp(lt[a] + w * (lt[a + 1] - lt[a]))
+            return n_star, t_star, t_star / t_learned
+    # beyond the finest plain level: extrapolate with the last segment's slopes
+    a = len(ns) - 2
+    w = (math.log(e_target) - le[a]) / (le[a + 1] - le[a])
+    n_star = math.exp(ln[a] + w * (ln[a + 1] - ln[a]))
+    t_star = math.exp(lt[a] + w * (lt[a + 1] - lt[a]))
+    return n_star, t_star, t_star / t_learned
+
+
+def evaluate(model, test_idx, truth64, ladder=(64, 128, 256), log=print):
+    res = {"test_instances": len(test_idx), "plain": {}}
+    for n in ladder:
+        fr, secs = plain_rollout(test_idx, n)
+        e = float(l1_density(fr, truth64)[:, 1:].mean())
+        res["plain"][n] = {"l1_rho": e, "seconds": secs}
+        log(f"  plain {n:4d}^2: L1(rho) {e:.5f}  wall {secs:.3f} s")
+    fr, secs, fell = learned_rollout(model, test_idx, log=log)
+    e = float(l1_density(fr, truth64)[:, 1:].mean())
+    res["learned64"] = {"l1_rho": e, "seconds": secs, "fallbacks": len(fell)}
+    log(f"  learned 64^2: L1(rho) {e:.5f}  wall {secs:.3f} s  fallbacks {len(fell)}")
+    ns = np.array(ladder, dtype=float)
+    errs = np.array([res["plain"][n]["l1_rho"] for n in ladder])
+    times = np.array([res["plain"][n]["seconds"] for n in ladder])
+    n_star, t_star, sp = matched_speedup(ns, errs, times, e, secs)
+    res["matched"] = {"plain_resolution": n_star, "plain_seconds": t_star, "speedup": sp}
+    log(f"  matched plain resolution {n_star:.1f}^2 ({t_star:.3f} s): learned is {sp:.2f}x faster")
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--train-traj", type=int, default=448)
+    ap.add_argument("--test-traj", type=int, default=64)
+    ap.add_argument("--iters", type=int, default=600)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--unroll", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    args = ap.parse_args(argv)
+    out = Path(args.out)
+    out.mkdir(parents=True, exist_ok=True)
+    logf = open(out / "learned_log.txt", "w")
+
+    def log(*a):
+        s = " ".join(str(x) for x in a)
+        print(s, flush=True)
+        logf.write(s + "\n")
+        logf.flush()
+
+    torch.cuda.set_device(0)
+    log(f"generating {args.train_traj} training trajectories")
+    train_idx = list(range(args.train_traj))
+    test_idx = list(range(100000, 100000 + args.test_traj))
+    data, gen_s = generate(train_idx, log=log)
+    data64 = torch.from_numpy(data[COARSE]).cuda()
+    log(f"training: {args.iters} iterations, batch {args.batch}, unroll {args.unroll}")
+    t0 = time.perf_counter()
+    model, hist = train(data64, iters=args.iters, batch=args.batch, unroll=args.unroll, lr=args.lr, log=log)
+    train_s = time.perf_counter() - t0
+    np.savez(out / "flux_corrector_riemann64.npz", **{k: v.numpy() for k, v in model.state().items()})
+    del data64
+    log("evaluation: generating held-out truth")
+    truth, _ = generate(test_idx, log=log)
+    res = evaluate(model, test_idx, truth[COARSE], log=log)
+    res.update({"train_trajectories": args.train_traj, "iters": args.iters, "batch": args.batch,
+                "unroll": args.unroll, "generate_seconds": gen_s, "train_seconds": train_s,
+                "loss_first": hist[0], "loss_last": float(np.mean(hist[-20:])),
+                "parameters": model.num_parameters()})
+    (out / "learned_eval.json").write_text(json.dumps(res, indent=1))
+    log(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,206 @@
+"""Learned flux correction: the differentiable torch step, the BF16-MFMA network kernels, and the
+corrected coarse solver on the HIP stage kernels.
+
+CPU: the torch restatement of the step equals the oracle bit for bit; corrections keep the update
+conservative; the float64 network oracle's gradients pass finite differences.
+GPU: MFMA operand layout; network forward and all gradients against the float64 oracle (BF16
+tolerances); directional finite differences of the kernels' own gradients; periodic translation
+equivariance; corrected HIP stage kernels equal the torch step with the same increments; a zero head
+reproduces the plain solver bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle2d
+
+
+def riemann_batch(n, idx):
+    from npde.families import riemann
+    return np.stack([riemann(k, n)[0] for k in idx])
+
+
+# ------------------------------------------------------------------------------------------- CPU
+@pytest.mark.parametrize("scheme,lim", [("hllc", "vanleer"), ("hllc", "minmod"), ("rusanov", "none"),
+                                        ("hllc", "unlimited")])
+def test_torch_step_equals_oracle(oracle, scheme, lim):
+    from npde.learned import torch_solver as ts
+    n = 40
+    U0 = riemann_batch(n, [0, 3])
+    g = oracle2d.Grid.make(n, n, bc_x="periodic", bc_y="periodic")
+    dt = 0.0015
+    out = ts.step(torch.from_numpy(U0), torch.full((2,), dt, dtype=torch.float64), 1 / n, 1 / n, scheme, lim)
+    for k in range(2):
+        a = U0[k].copy()
+        oracle.step(g, scheme, lim, a, dt)
+        assert np.array_equal(out[k].numpy(), a)
+
+
+def test_torch_advance_lands_like_oracle(oracle):
+    from npde.learned import torch_solver as ts
+    n = 32
+    U0 = riemann_batch(n, [5])
+    g = oracle2d.Grid.make(n, n, bc_x="periodic", bc_y="periodic")
+    a = U0[0].copy()
+    r = oracle.run(g, "hllc", "vanleer", 0.5, a, 0.04)
+    U, t, _ = ts.advance(torch.from_numpy(U0), torch.zeros(1, dtype=torch.float64),
+                         torch.full((1,), 0.04, dtype=torch.float64), 1 / n, 1 / n)
+    assert float(t[0]) == r["t"]
+    assert np.array_equal(U[0].numpy(), a)
+
+
+def test_corrections_are_conservative_and_differentiable():
+    from npde.learned import torch_solver as ts
+    n = 24
+    U0 = torch.from_numpy(riemann_batch(n, [1]))
+    cx = (0.01 * torch.randn(1, 4, n, n, dtype=torch.float64)).requires_grad_()
+    cy = (0.01 * torch.randn(1, 4, n, n, dtype=torch.float64)).requires_grad_()
+    U1 = ts.step(U0, torch.full((1,), 0.002, dtype=torch.float64), 1 / n, 1 / n, cx_corr=cx, cy_corr=cy)
+    tot0, tot1 = U0.sum(dim=(-1, -2)), U1.sum(dim=(-1, -2))
+    assert torch.allclose(tot0, tot1, rtol=0, atol=1e-12 * float(tot0.abs().max()))
+    (U1[:, 0] ** 2).sum().backward()
+    assert torch.isfinite(cx.grad).all() and cx.grad.abs().sum() > 0 and cy.grad.abs().sum() > 0
+
+
+def test_network_oracle_gradients_pass_finite_differences():
+    import nn_oracle
+    g = torch.Generator().manual_seed(0)
+    x = torch.randn(1, 6, 8, 3, generator=g, dtype=torch.float64)
+    ws = [torch.randn(4, 3, 3, 3, generator=g, dtype=torch.float64, requires_grad=True),
+          torch.randn(2, 4, 3, 3, generator=g, dtype=torch.float64, requires_grad=True)]
+    bs = [torch.randn(4, generator=g, dtype=torch.float64, requires_grad=True),
+          torch.randn(2, generator=g, dtype=torch.float64, requires_grad=True)]
+    f = lambda w0, b0, w1, b1: nn_oracle.forward(x, [w0, w1], [b0, b1], round_bf16=False)
+    assert torch.autograd.gradcheck(f, (ws[0], bs[0], ws[1], bs[1]), eps=1e-6, atol=1e-6)
+
+
+def test_corrector_shape_and_identity_at_init():
+    from npde.learned.model import FluxCorrector
+    m = FluxCorrector()
+    assert m.num_parameters() == 32392
+    assert all(float(w.abs().max()) == 0.0 for w in [m.weights[-1]])
+
+
+# ------------------------------------------------------------------------------------------- GPU
+@pytest.fixture(scope="module")
+def cuda(npde_mod):
+    return torch.device("cuda:0")
+
+
+@pytest.mark.gpu
+def test_mfma_operand_layout(cuda):
+    import ctypes as C
+    import npde._lib as L
+    lib = L.load()
+    a = torch.arange(256, dtype=torch.float32, device=cuda).reshape(16, 16) % 7 - 3
+    b = torch.arange(256, dtype=torch.float32, device=cuda).reshape(16, 16) % 5 - 2
+    d = torch.empty(16, 16, dtype=torch.float32, device=cuda)
+    rc = lib.npde_mfma_probe(C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), C.c_void_p(d.data_ptr()),
+                             C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert torch.equal(d, a @ b)  # small integers: exact in bf16 and in fp32 accumulation
+
+
+def _random_net(seed, cuda, head_scale=0.3):
+    from npde.learned.model import FluxCorrector
+    m = FluxCorrector(seed=seed).to(cuda)
+    torch.manual_seed(seed)
+    with torch.no_grad():
+        m.weights[-1].copy_(torch.randn_like(m.weights[-1]) * head_scale)
+        for b in m.biases:
+            b.copy_(torch.randn_like(b) * 0.1)
+    return m
+
+
+@pytest.mark.gpu
+def test_network_forward_and_gradients_match_oracle(cuda):
+    import nn_oracle
+    from npde.learned.conv import FluxCNN
+    m = _random_net(1, cuda)
+    x = torch.randn(2, 16, 32, 16, device=cuda, generator=torch.Generator(device=cuda).manual_seed(1))
+    x[..., 8:] = 0
+    x.requires_grad_(True)
+    params = [p.detach().clone().requires_grad_(True) for p in m.params()]
+    out = FluxCNN.apply(x, *params)
+    gout = torch.randn_like(out)
+    (out * gout).sum().backward()
+    xo = x.detach().double().cpu().requires_grad_(True)
+    po = [p.detach().double().cpu().requires_grad_(True) for p in params]
+    ref = nn_oracle.forward(xo, po[0::2], po[1::2])
+    (ref * gout.double().cpu()).sum().backward()
+    err = (out.detach().double().cpu() - ref.detach()).abs().max() / ref.abs().max()
+    assert err < 2e-3, err
+    for got, want in [(x.grad, xo.grad)] + [(p.grad, q.grad) for p, q in zip(params, po)]:
+        e = (got.double().cpu() - want).abs().max() / want.abs().max().clamp_min(1e-30)
+        assert e < 3e-2, e
+
+
+@pytest.mark.gpu
+def test_network_gradients_pass_directional_finite_differences(cuda):
+    from npde.learned.conv import FluxCNN
+    m = _random_net(2, cuda)
+    x = torch.randn(1, 16, 16, 16, device=cuda, generator=torch.Generator(device=cuda).manual_seed(2))
+    x[..., 8:] = 0
+    params = [p.detach().clone().requires_grad_(True) for p in m.params()]
+    gout = torch.randn(1, 16, 16, 8, device=cuda, generator=torch.Generator(device=cuda).manual_seed(4))
+    (FluxCNN.apply(x, *params) * gout).sum().backward()
+    gen = torch.Generator(device=cuda).manual_seed(3)
+    for k in (0, 2, 8):
+        # half along the gradient, half random: the difference quotient of bf16-rounded weights carries
+        # rounding noise ~ |grad| * ulp, so the direction must not be nearly orthogonal to the gradient
+        r = torch.randn(params[k].shape, device=cuda, generator=gen)
+        v = params[k].grad / params[k].grad.norm() + r / r.norm()
+        v /= v.norm()
+        eps = 0.05 * float(params[k].detach().norm()) + 1e-3
+
+        def f(s):
+            ps = [p.detach().clone() for p in params]
+            ps[k] += s * v
+            return float((FluxCNN.apply(x, *ps) * gout).sum())
+
+        fd = (f(eps) - f(-eps)) / (2 * eps)
+        an = float((params[k].grad * v).sum())
+        assert abs(fd - an) <= 0.05 * abs(an) + 1e-3, (k, fd, an)
+
+
+@pytest.mark.gpu
+def test_network_is_translation_equivariant(cuda):
+    m = _random_net(4, cuda)
+    U = torch.from_numpy(riemann_batch(32, [2])).to(cuda)
+    cx, cy = m(U)
+    cx2, cy2 = m(torch.roll(U, shifts=(16, 16), dims=(-2, -1)))
+    assert torch.equal(torch.roll(cx, shifts=(16, 16), dims=(-2, -1)), cx2)
+    cx3, _ = m(torch.roll(U, shifts=(1, 3), dims=(-2, -1)))
+    assert torch.allclose(torch.roll(cx, shifts=(1, 3), dims=(-2, -1)), cx3, rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.gpu
+def test_corrected_hip_step_equals_torch_step(npde_mod, cuda):
+    from npde.learned import torch_solver as ts
+    n, B = 64, 3
+    U0 = torch.from_numpy(riemann_batch(n, range(B))).to(cuda)
+    cx = (0.02 * torch.randn(B, 4, n, n, dtype=torch.float64, device=cuda)).contiguous()
+    cy = (0.02 * torch.randn(B, 4, n, n, dtype=torch.float64, device=cuda)).contiguous()
+    grids = [npde_mod.make_grid(n, n, bc_x=1, bc_y=1)] * B
+    dt = torch.full((B,), 0.001, dtype=torch.float64, device=cuda)
+    ref = ts.step(U0, dt, 1 / n, 1 / n, cx_corr=cx, cy_corr=cy)
+    plain = ts.step(U0, dt, 1 / n, 1 / n)
+    torch.cuda.synchronize()
+    with npde_mod.Batch(grids, "hllc", "vanleer", 0.5) as b:
+        for k in range(B):
+            b.set_state(k, U0[k].cpu().numpy())
+            b.set_flux_correction(k, cx[k].data_ptr(), cy[k].data_ptr())
+        b.step_all([0.001] * B)
+        got = np.stack([b.get_state(k) for k in range(B)])
+        zero = torch.zeros_like(cx)
+        for k in range(B):
+            b.set_state(k, U0[k].cpu().numpy())
+            b.set_flux_correction(k, zero[k].data_ptr(), zero[k].data_ptr())
+        torch.cuda.synchronize()
+        b.step_all([0.001] * B)
+        got0 = np.stack([b.get_state(k) for k in range(B)])
+    assert np.array_equal(got, ref.cpu().numpy())
+    assert np.array_equal(got0, plain.cpu().numpy())   # zero increments: the plain solver's bits
+    tot0 = U0.sum(dim=(-1, -2)).cpu().numpy()
+    assert np.allclose(got.sum(axis=(-1, -2)), tot0, rtol=0, atol=1e-11 * np.abs(tot0).max())
