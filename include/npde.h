@@ -53,7 +53,7 @@ typedef struct {
 
 /* Kernel launch configuration (chosen by npde_autotune or set by hand). */
 typedef struct {
-    int rows_per_task;    /* band height of one wavefront's task (>= 1) */
+    int rows_per_task;    /* band height of one wavefront's task; 0 = automatic (4..64, enough tasks to fill the GPU) */
     int waves_per_block;  /* wavefronts per workgroup: 1, 2, 4 (or 8 within the 256-lane bound) */
     int steps_per_graph;  /* RK2 steps captured per hipGraph launch in npde_run / npde_advance_to */
     int use_graph;        /* 0: plain stream launches */
@@ -79,6 +79,11 @@ npde_status npde_get_state(npde_solver* s, int instance, double* U, int layout);
 npde_status npde_set_state_device(npde_solver* s, int instance, const double* dU);
 npde_status npde_get_state_device(npde_solver* s, int instance, double* dU);
 npde_status npde_get_time(npde_solver* s, int instance, double* t, long long* steps);
+/* Where the library keeps an instance's state: *U = device address of its 4 planes, *plane = elements
+ * between planes (nx*ny rounded up to a multiple of 64), *instance_stride = elements from this instance's
+ * state to the next instance's.  For callers that read (or, between calls, write) the state in place
+ * without copies; valid until npde_destroy.  No counterpart in the reference (its U is host memory). */
+npde_status npde_state_pointer(npde_solver* s, int instance, double** U, long long* plane, long long* instance_stride);
 
 /* ---- the hot path: Solver::compute_dt (solver2d.hpp:67-83) and Solver::step (solver2d.hpp:86-103) ---- */
 /* dt of every instance (array of n); on a non-physical state returns NPDE_ERR_NONPHYSICAL and fills bad. */

@@ -65,7 +65,7 @@ struct npde_solver {
     bool has_corr = false;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    npde_tuning tune{64, 4, 32, 1};
+    npde_tuning tune{0, 4, 32, 1};
     hipGraphExec_t graph = nullptr;
     int graph_key_frames = -1;  // nftasks the cached graph was captured with (0: no frame kernel)
     int graph_key_steps = 0;
@@ -102,9 +102,24 @@ bool valid_grid(const npde_grid_desc& g, std::string& why) {
     return true;
 }
 
+// Band height: the tuned value, or (rows_per_task <= 0) the tallest band in [4, 64] that still gives
+// about 4096 tasks, so that small grids are not bound by one wavefront marching many rows.
+int band_rows(const npde_solver* s) {
+    if (s->tune.rows_per_task > 0) return s->tune.rows_per_task;
+    long long cols = 0, rows_total = 0;
+    for (const auto& g : s->grids) {
+        cols += (g.nx + kStripCols - 1) / kStripCols;
+        rows_total += (long long)((g.nx + kStripCols - 1) / kStripCols) * g.ny;
+    }
+    (void)cols;
+    int r = 64;
+    while (r > 4 && rows_total / r < 4096) r /= 2;
+    return r;
+}
+
 void build_tasks(npde_solver* s) {
     s->htasks.clear();
-    const int rows = std::max(1, s->tune.rows_per_task);
+    const int rows = band_rows(s);
     for (int k = 0; k < s->n; ++k) {
         InstDesc& D = s->hdesc[k];
         D.task0 = int(s->htasks.size());
@@ -500,6 +515,16 @@ npde_status npde_get_time(npde_solver* s, int inst, double* t, long long* steps)
     return NPDE_OK;
 }
 
+npde_status npde_state_pointer(npde_solver* s, int inst, double** U, long long* plane, long long* instance_stride) {
+    if (!s || inst < 0 || inst >= s->n || !U) return fail(s, NPDE_ERR_INVALID, "bad arguments");
+    *U = s->hdesc[inst].U;
+    if (plane) *plane = s->hdesc[inst].plane;
+    if (instance_stride)
+        *instance_stride = inst + 1 < s->n ? (long long)(s->state_off[inst + 1] - s->state_off[inst])
+                                           : 8 * s->hdesc[inst].plane;
+    return NPDE_OK;
+}
+
 npde_status npde_compute_dt(npde_solver* s, double* dt, npde_bad_cell* bad) {
     if (!s || !dt) return fail(s, NPDE_ERR_INVALID, "null argument");
     npde_status st = set_host_mode(s);
@@ -595,6 +620,10 @@ npde_status npde_run_trajectory(npde_solver* s, double t_start, double t_end, in
             total += size_t(nout + 1) * 4 * m * (g.ny / f);
         }
     }
+    // the same levels and frame count as the last run: keep its buffers, frame tasks and captured graph
+    if (s->dftasks && s->dftimes && s->traj_nout == nout && int(s->traj_sizes.size()) == n_sizes &&
+        std::equal(s->traj_sizes.begin(), s->traj_sizes.end(), sizes))
+        return auto_run(s, t_start, t_end, nout, true, bad);
     if (s->dtraj) (void)hipFree(s->dtraj);
     s->dtraj = nullptr;
     s->traj_bytes = total * sizeof(float);
@@ -704,7 +733,7 @@ npde_status npde_get_tuning(const npde_solver* s, npde_tuning* t) {
 
 npde_status npde_set_tuning(npde_solver* s, const npde_tuning* t) {
     if (!s || !t) return NPDE_ERR_INVALID;
-    if (t->rows_per_task < 1 || !(t->waves_per_block == 1 || t->waves_per_block == 2 || t->waves_per_block == 4) ||
+    if (t->rows_per_task < 0 || !(t->waves_per_block == 1 || t->waves_per_block == 2 || t->waves_per_block == 4) ||
         t->steps_per_graph < 1)
         return fail(s, NPDE_ERR_INVALID, "bad tuning");
     const bool rows_changed = t->rows_per_task != s->tune.rows_per_task;
@@ -859,11 +888,11 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
     double best_ms = 1e30;
     int max_ny = 1;
     for (const auto& g : s->grids) max_ny = std::max(max_ny, g.ny);
-    const int rows_c[] = {16, 32, 64, 128, 256};
+    const int rows_c[] = {4, 8, 16, 32, 64, 128};
     const int wpb_c[] = {1, 2, 4};
     for (int r : rows_c) {
         if (st != NPDE_OK) break;
-        if (r > 2 * max_ny && r != rows_c[0]) continue;
+        if (r > max_ny && r != rows_c[0]) continue;
         for (int w : wpb_c) {
             npde_tuning t = orig;
             t.rows_per_task = r;

@@ -65,7 +65,8 @@ class FluxCorrector(torch.nn.Module):
 
     def forward(self, U: torch.Tensor):
         """(B, 4, H, W) float64 state -> (corr_x, corr_y), each (B, 4, H, W) float64."""
-        out = FluxCNN.apply(features(U), *self.params())  # (B, H, W, 8) float32
+        # the network's input is not differentiated: gradients reach it through the flux increments only
+        out = FluxCNN.apply(features(U.detach()), *self.params())  # (B, H, W, 8) float32
         out = (out * self.out_scale).to(torch.float64)
         cx = out[..., 0:4].permute(0, 3, 1, 2).contiguous()
         cy = out[..., 4:8].permute(0, 3, 1, 2).contiguous()

@@ -3,10 +3,12 @@
 The learned coarse solver runs on the HIP stage kernels (flux increments added inside the kernel).
 Training needs gradients of a rollout loss through solver steps with respect to those increments;
 this module provides the same step as torch operations so autograd can supply them.  Every
-expression keeps the reference's operation order (state2d.hpp, limiters.hpp, riemann2d.hpp,
-solver2d.hpp:86-180), so on the same input it reproduces the kernels' results bit for bit (checked in
-tests/test_learned.py against the CPU oracle).  Unselected branches use harmless denominators so that
-their gradients stay finite; the selected values are unchanged.
+element's expression keeps the reference's operation order (state2d.hpp, limiters.hpp,
+riemann2d.hpp, solver2d.hpp:86-180), so on the same input it reproduces the kernels' results bit for
+bit (checked in tests/test_learned.py against the CPU oracle and the HIP kernels).  Operations are
+vectorised over the four variables and over both face directions (x and y faces solved in one
+Riemann call), which keeps the number of launched ops per step small.  Unselected branches use
+harmless denominators so their gradients stay finite; the selected values are unchanged.
 
 State: (B, 4, ny, nx) float64 on a periodic grid.  Corrections: (B, 4, ny, nx) each, cx on the face
 left of a cell and cy on the face below it, added to the face fluxes in both RK stages.
@@ -75,22 +77,14 @@ def limit(kind: str, a, b):
     return torch.where(ab <= 0.0, torch.zeros_like(a), 2.0 * a * b / den)
 
 
-def side(w, dw, h):
-    """face-side state at W + h*dW: to_conserved then to_primitive, plus sound speed (x frame)."""
-    d = w[0] + h * dw[0]
-    u = w[1] + h * dw[1]
-    v = w[2] + h * dw[2]
-    p = w[3] + h * dw[3]
+def sides(E):
+    """E (..., 4, H, W) face values of (rho, u, v, p) -> to_conserved, to_primitive, sound speed,
+    as a dict of (..., H, W) tensors in the x frame (mn = rho*u, un = u)."""
+    d, u, v, p = E.unbind(-3)
     kin = (0.5 * d) * (u * u + v * v)
     m, n, e = d * u, d * v, p / _c(G1, p) + kin
     _, u2, v2, p2 = prim(d, m, n, e)
-    a = _sqrt((GAMMA * p2) / d)
-    return {"d": d, "mn": m, "mt": n, "e": e, "un": u2, "ut": v2, "p": p2, "a": a}
-
-
-def y_frame(s):
-    return {"d": s["d"], "mn": s["mt"], "mt": s["mn"], "e": s["e"], "un": s["ut"], "ut": s["un"], "p": s["p"],
-            "a": s["a"]}
+    return {"d": d, "mn": m, "mt": n, "e": e, "un": u2, "ut": v2, "p": p2, "a": _sqrt((GAMMA * p2) / d)}
 
 
 def _phys(s):
@@ -126,42 +120,37 @@ def rusanov(A, B):
     return tuple(0.5 * (FA[k] + FB[k]) - h * (UB[k] - UA[k]) for k in range(4))
 
 
-def _shift(x, k, dim):
-    """value of the cell k places back along dim (periodic): out[i] = x[i - k]"""
-    return torch.roll(x, shifts=k, dims=dim)
+_YSWAP = {"d": "d", "mn": "mt", "mt": "mn", "e": "e", "un": "ut", "ut": "un", "p": "p", "a": "a"}
 
 
 def rhs(U, dx, dy, scheme="hllc", lim="vanleer", cx_corr=None, cy_corr=None):
     """L(U) for a periodic batch (B, 4, ny, nx); x faces first, then y (solver2d.hpp:129-180)."""
     riem = hllc if scheme == "hllc" else rusanov
-    w = prim(U[:, 0], U[:, 1], U[:, 2], U[:, 3])
-    cx, cy = 1.0 / dx, 1.0 / dy
-    # x direction (dim -1)
-    wm = [_shift(q, 1, -1) for q in w]
-    wp = [_shift(q, -1, -1) for q in w]
-    dws = [limit(lim, w[k] - wm[k], wp[k] - w[k]) for k in range(4)]
-    L_ = side(w, dws, -0.5)
-    R_ = side(w, dws, +0.5)
-    Rprev = {k: _shift(v, 1, -1) for k, v in R_.items()}
-    f = riem(Rprev, L_)                                   # face left of each cell: (d, mn=mx, mt=my, e)
-    fl = [f[0], f[1], f[2], f[3]]
+    d, u, v, p = prim(U[:, 0], U[:, 1], U[:, 2], U[:, 3])
+    W = torch.stack([d, u, v, p], dim=1)                                   # (B, 4, H, W)
+    W2 = torch.stack([W, W])                                               # dir x, dir y
+    Wm = torch.stack([torch.roll(W, 1, -1), torch.roll(W, 1, -2)])         # cell i-1 / j-1
+    Wp = torch.stack([torch.roll(W, -1, -1), torch.roll(W, -1, -2)])       # cell i+1 / j+1
+    dW = limit(lim, W2 - Wm, Wp - W2)
+    h = torch.tensor([-0.5, 0.5], dtype=U.dtype, device=U.device).view(2, 1, 1, 1, 1, 1)
+    S = sides(W2.unsqueeze(0) + h * dW.unsqueeze(0))                       # (sign, dir, B, H, W) fields
+    # face left of / below each cell: A = right/top side of the previous cell, B = left/bottom side here
+    A, Bs = {}, {}
+    for k in S:
+        ax = torch.roll(S[k][1, 0], 1, -1)
+        ay = torch.roll(S[_YSWAP[k]][1, 1], 1, -2)
+        A[k] = torch.stack([ax, ay])
+        Bs[k] = torch.stack([S[k][0, 0], S[_YSWAP[k]][0, 1]])
+    f = riem(A, Bs)                                                        # normal frame, (dir, B, H, W)
+    Fl = torch.stack([f[0][0], f[1][0], f[2][0], f[3][0]], dim=1)          # x faces: (rho, mx, my, E)
+    Gb = torch.stack([f[0][1], f[2][1], f[1][1], f[3][1]], dim=1)          # y faces back to (rho, mx, my, E)
     if cx_corr is not None:
-        fl = [fl[q] + cx_corr[:, q] for q in range(4)]
-    fr = [_shift(q, -1, -1) for q in fl]
-    Lx = [(0.0 + cx * fl[q]) - cx * fr[q] for q in range(4)]
-    # y direction (dim -2)
-    wm = [_shift(q, 1, -2) for q in w]
-    wp = [_shift(q, -1, -2) for q in w]
-    dws = [limit(lim, w[k] - wm[k], wp[k] - w[k]) for k in range(4)]
-    Bo = y_frame(side(w, dws, -0.5))
-    To = y_frame(side(w, dws, +0.5))
-    Tprev = {k: _shift(v, 1, -2) for k, v in To.items()}
-    g = riem(Tprev, Bo)                                   # y frame: (d, mn=my, mt=mx, e)
-    gb = [g[0], g[2], g[1], g[3]]                         # back to (rho, mx, my, E): face below each cell
+        Fl = Fl + cx_corr
     if cy_corr is not None:
-        gb = [gb[q] + cy_corr[:, q] for q in range(4)]
-    gt = [_shift(q, -1, -2) for q in gb]
-    return torch.stack([(Lx[q] + cy * gb[q]) - cy * gt[q] for q in range(4)], dim=1)
+        Gb = Gb + cy_corr
+    cx, cy = 1.0 / dx, 1.0 / dy
+    Lx = (0.0 + cx * Fl) - cx * torch.roll(Fl, -1, -1)
+    return (Lx + cy * Gb) - cy * torch.roll(Gb, -1, -2)
 
 
 def wavespeed_dt(U, dx, dy, cfl=0.5):
@@ -169,10 +158,8 @@ def wavespeed_dt(U, dx, dy, cfl=0.5):
     with torch.no_grad():
         d, u, v, p = prim(U[:, 0], U[:, 1], U[:, 2], U[:, 3])
         a = _sqrt((GAMMA * p) / d)
-        sx = (u.abs() + a).flatten(1).amax(1)
-        sy = (v.abs() + a).flatten(1).amax(1)
-        sx = torch.clamp(sx, min=0.0)
-        sy = torch.clamp(sy, min=0.0)
+        sx = torch.clamp((u.abs() + a).flatten(1).amax(1), min=0.0)
+        sy = torch.clamp((v.abs() + a).flatten(1).amax(1), min=0.0)
         return torch.full_like(sx, cfl) / (sx / _c(dx, sx) + sy / _c(dy, sy))
 
 

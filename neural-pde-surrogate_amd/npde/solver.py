@@ -192,6 +192,27 @@ class Batch:
     def get_state_device(self, inst: int, ptr: int):
         check(self._h, self._lib.npde_get_state_device(self._h, inst, C.c_void_p(ptr)))
 
+    def state_view(self):
+        """Zero-copy torch view (n, 4, ny, nx) of every instance's state in device memory (all instances
+        must have the same size).  Read it after a call returns; write it only between calls."""
+        import torch
+        g0 = self.grids[0]
+        if any((g.nx, g.ny) != (g0.nx, g0.ny) for g in self.grids):
+            raise ValueError("state_view needs instances of one size")
+        ptr, plane, stride = C.c_void_p(), C.c_longlong(), C.c_longlong()
+        check(self._h, self._lib.npde_state_pointer(self._h, 0, C.byref(ptr), C.byref(plane), C.byref(stride)))
+        if self.n > 1:
+            p1, s1 = C.c_void_p(), C.c_longlong()
+            check(self._h, self._lib.npde_state_pointer(self._h, 1, C.byref(p1), C.byref(s1), None))
+            stride = C.c_longlong((p1.value - ptr.value) // 8)
+
+        class _View:
+            __cuda_array_interface__ = {
+                "shape": (self.n, 4, g0.ny, g0.nx), "typestr": "<f8", "data": (ptr.value, False), "version": 3,
+                "strides": (stride.value * 8, plane.value * 8, g0.nx * 8, 8)}
+
+        return torch.as_tensor(_View(), device="cuda")
+
     def time(self, inst: int = 0):
         t, steps = C.c_double(), C.c_longlong()
         check(self._h, self._lib.npde_get_time(self._h, inst, C.byref(t), C.byref(steps)))
