@@ -2,7 +2,7 @@
 
 A stack of 3x3 periodic convolutions (4 hidden layers of 32 channels with tanh-GELU, then a linear
 head; 32,392 parameters) maps the coarse state to 8 per-face flux increments: 4 conserved components
-on the face left of each cell and 4 on the face below it.  The increments are added to the solver's
+on the face left of each cell and 4 on the face below it, each bounded by out_scale * tanh.  The increments are added to the solver's
 face fluxes (HLLC + van Leer) inside the stage kernels, so the update stays in flux form: mass,
 momentum and energy are conserved to rounding whatever the network outputs.  The head is zero
 initialised, so an untrained corrector reproduces the plain solver exactly (analog: the zero-init
@@ -67,7 +67,8 @@ class FluxCorrector(torch.nn.Module):
         """(B, 4, H, W) float64 state -> (corr_x, corr_y), each (B, 4, H, W) float64."""
         # the network's input is not differentiated: gradients reach it through the flux increments only
         out = FluxCNN.apply(features(U.detach()), *self.params())  # (B, H, W, 8) float32
-        out = (out * self.out_scale).to(torch.float64)
+        # bounded increments (|dF| < out_scale per face): a rollout cannot be driven far off by one call
+        out = (torch.tanh(out) * self.out_scale).to(torch.float64)
         cx = out[..., 0:4].permute(0, 3, 1, 2).contiguous()
         cy = out[..., 4:8].permute(0, 3, 1, 2).contiguous()
         return cx, cy

@@ -101,7 +101,7 @@ def train(data: torch.Tensor, iters=600, batch=32, unroll=8, lr=1e-3, warmup=50,
         k0 = torch.randint(0, F - 1 - nu + 1, (batch,), generator=gen)
         U = data[traj, k0].to(torch.float64)
         t = tf[k0.to(dev)]
-        loss = 0.0
+        loss, skip = 0.0, False
         for j in range(1, nu + 1):
             tn = tf[(k0 + j).to(dev)]
 
@@ -120,13 +120,15 @@ def train(data: torch.Tensor, iters=600, batch=32, unroll=8, lr=1e-3, warmup=50,
                 badk = torch.nonzero(~torch.isfinite(err)).flatten().tolist()
                 log(f"  iter {it:4d}  interval {j}: non-finite state for trajectories "
                     f"{[int(traj[b]) for b in badk][:6]} from frames {[int(k0[b]) for b in badk][:6]}")
+                skip = True
                 break
             loss = loss + err.mean()
-        loss = loss / nu
         opt.zero_grad(set_to_none=True)
-        if not torch.isfinite(loss):  # a corrected rollout left the physical states: skip the batch
+        # a corrected rollout that left the physical states poisons every gradient: skip the batch
+        if skip or not bool(torch.isfinite(loss)):
             sched.step()
             continue
+        loss = loss / nu
         loss.backward()
         gnorm = torch.nn.utils.clip_grad_norm_(model.parameters(), 1.0)
         if torch.isfinite(gnorm):
@@ -134,7 +136,7 @@ def train(data: torch.Tensor, iters=600, batch=32, unroll=8, lr=1e-3, warmup=50,
         sched.step()
         hist.append(float(loss.detach()))
         if it % 25 == 0 or it == iters - 1:
-            log(f"  iter {it:4d}  loss {float(loss):.5f}  |g| {float(gnorm):.3g}  lr {sched.get_last_lr()[0]:.2e}  "
+            log(f"  iter {it:4d}  loss {float(loss.detach()):.5f}  |g| {float(gnorm):.3g}  lr {sched.get_last_lr()[0]:.2e}  "
                 f"({time.perf_counter() - t0:.0f} s)")
     return model, hist
 
