@@ -36,6 +36,11 @@ typedef enum {
     NPDE_LIM_NONE = 0, NPDE_LIM_UNLIMITED = 1, NPDE_LIM_MINMOD = 2, NPDE_LIM_VANLEER = 3
 } npde_limiter;                                                                 /* limiters.hpp:5 */
 typedef enum { NPDE_LAYOUT_PLANES = 0, NPDE_LAYOUT_GHOSTED = 1 } npde_layout;
+/* Arithmetic of the stage kernels.  EXACT (default): every operation is the IEEE binary64 one the
+ * reference performs, in its order, without contraction -- results are bit-identical to the reference
+ * solver.  FAST: reciprocal-based divisions, a shortened square root and fused multiply-adds; each
+ * step differs from EXACT by rounding-level amounts.  No counterpart in the reference. */
+typedef enum { NPDE_ARITH_EXACT = 0, NPDE_ARITH_FAST = 1 } npde_arith;
 
 /* Grid of one instance: replaces make_grid(nx, ny, x_min, x_max, y_min, y_max, bc_x, bc_y)
  * (grid2d.hpp:30-39). */
@@ -149,6 +154,9 @@ npde_status npde_set_tuning(npde_solver* s, const npde_tuning* t);
 /* Time candidate launch configurations on this handle's own instances (state is saved and restored)
  * and keep the fastest; results are cached per (sizes, bc, flux, limiter) in cache_path if non-NULL. */
 npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* chosen);
+/* Select the stage kernels' arithmetic (npde_arith) for the following steps of this handle. */
+npde_status npde_set_arithmetic(npde_solver* s, int arith);
+npde_status npde_get_arithmetic(const npde_solver* s, int* arith);
 /* Average device milliseconds of the stage-1 and stage-2 kernel launches over nsteps host-driven
  * steps of the current state (HIP events on the handle's stream around each launch; the state advances). */
 npde_status npde_time_stages(npde_solver* s, int nsteps, double* ms_stage1, double* ms_stage2);

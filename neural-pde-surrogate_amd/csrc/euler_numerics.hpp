@@ -13,6 +13,13 @@
 // two precomputed sides.  For a y face the normal frame is the x frame with the momenta and the
 // velocities relabelled; the relabelling is exact because (u*u + v*v) == (v*v + u*u) in IEEE
 // arithmetic, which is all the reference's swap_xy changes.
+//
+// Arithmetic policy: by default every operation is the IEEE one the reference performs (the `exact`
+// namespace).  A translation unit that defines NPDE_FAST_ARITH before including this header gets the
+// `fast` namespace instead: divisions by a reciprocal refined by two Newton steps without the final
+// rounding correction, a square root without its last correction step, and (with -ffp-contract=fast
+// on that unit) fused multiply-adds.  Each such operation is within ~2 ulp of the IEEE result, so a
+// step differs from the exact one by rounding-level amounts; it is a separate, opt-in variant.
 #pragma once
 
 #if defined(__HIPCC__)
@@ -23,7 +30,14 @@
 #define NPDE_HD inline
 #endif
 
+#if defined(NPDE_FAST_ARITH)
+#define NPDE_ARITH_NS fast
+#else
+#define NPDE_ARITH_NS exact
+#endif
+
 namespace npde {
+inline namespace NPDE_ARITH_NS {
 
 constexpr double kGamma = 1.4;
 constexpr double kGm1 = kGamma - 1.0;  // 0.3999999999999999 (1.4 is not exact in binary64)
@@ -64,7 +78,17 @@ NPDE_HD double tmax(double a, double b) { return (a < b) ? b : a; }
 // physical state produces -- this is the compiler's own instruction sequence, hence the same IEEE
 // quotient; the remainder is taken as y*q0 - x so that x = -0 still gives -0.
 // Define NPDE_IEEE_DIV to use the plain operator everywhere.
-#if defined(__HIP_DEVICE_COMPILE__) && !defined(NPDE_IEEE_DIV)
+#if defined(__HIP_DEVICE_COMPILE__) && defined(NPDE_FAST_ARITH)
+NPDE_HD double recip(double y) {
+    double r = __builtin_amdgcn_rcp(y);
+    double e = __builtin_fma(-y, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-y, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+NPDE_HD double div_r(double x, double, double r) { return x * r; }
+NPDE_HD double ddiv(double x, double y) { return x * recip(y); }
+#elif defined(__HIP_DEVICE_COMPILE__) && !defined(NPDE_IEEE_DIV)
 NPDE_HD double recip(double y) {
     double r = __builtin_amdgcn_rcp(y);
     double e = __builtin_fma(-y, r, 1.0);
@@ -77,9 +101,11 @@ NPDE_HD double div_r(double x, double y, double r) {
     const double rem = __builtin_fma(y, q0, -x);
     return __builtin_fma(-rem, r, q0);
 }
+NPDE_HD double ddiv(double x, double y) { return x / y; }
 #else
 NPDE_HD double recip(double y) { return y; }  // the plain path divides by y itself
 NPDE_HD double div_r(double x, double y, double) { return x / y; }
+NPDE_HD double ddiv(double x, double y) { return x / y; }
 #endif
 
 // reciprocal of gamma - 1 for div_r(x, kGm1, rgm1()); a pure function, hoisted by the compiler
@@ -90,7 +116,20 @@ NPDE_HD double rgm1() { return recip(kGm1); }
 // x itself for +-0 and +inf.  The sound speed's argument gamma*p/rho is never that small for a state
 // with p > 0, so the core runs without the scaling (the fix-up is kept); for x >= 2^-767, zeros,
 // infinities, negatives and NaNs this is the compiler's sequence, hence the IEEE result.
-#if defined(__HIP_DEVICE_COMPILE__) && !defined(NPDE_IEEE_DIV)
+// The fast variant stops before the last correction step.
+#if defined(__HIP_DEVICE_COMPILE__) && defined(NPDE_FAST_ARITH)
+NPDE_HD double sqrt_core(double x) {
+    const double g = __builtin_amdgcn_rsq(x);
+    double s = x * g;
+    double h = g * 0.5;
+    const double r = __builtin_fma(-h, s, 0.5);
+    s = __builtin_fma(s, r, s);
+    h = __builtin_fma(h, r, h);
+    const double d = __builtin_fma(-s, s, x);
+    s = __builtin_fma(d, h, s);
+    return __builtin_amdgcn_class(x, 0x260 /* -0, +0, +inf */) ? x : s;
+}
+#elif defined(__HIP_DEVICE_COMPILE__) && !defined(NPDE_IEEE_DIV)
 NPDE_HD double sqrt_core(double x) {
     const double g = __builtin_amdgcn_rsq(x);
     double s = x * g;
@@ -135,7 +174,7 @@ NPDE_HD double limit(double a, double b) {
         const double s = (dabs(a) < dabs(b)) ? a : b;
         return (a * b <= 0.0) ? 0.0 : s;
     } else {
-        const double s = 2.0 * a * b / (a + b);  // harmonic mean; unused where a*b <= 0
+        const double s = ddiv(2.0 * a * b, a + b);  // harmonic mean; unused where a*b <= 0
         return (a * b <= 0.0) ? 0.0 : s;
     }
 }
@@ -179,16 +218,16 @@ NPDE_HD Flux hllc(const Side& A, const Side& B) {
     const double sr = tmax(A.un + A.a, B.un + B.a);
     const double num = ((B.p - A.p) + (A.d * A.un) * (sl - A.un)) - (B.d * B.un) * (sr - B.un);
     const double den = A.d * (sl - A.un) - B.d * (sr - B.un);
-    const double ss = num / den;
+    const double ss = ddiv(num, den);
     const bool left = (0.0 <= sl) || (0.0 <= ss);
     const bool star = left ? !(0.0 <= sl) : (0.0 <= sr);
     const Side X = pick(left, A, B);
     const double S = left ? sl : sr;
     const Flux F = phys_flux(X);
     const double q = X.d * (S - X.un);
-    const double f = q / (S - ss);
+    const double f = ddiv(q, S - ss);
     const double sd = f, smn = f * ss, smt = f * X.ut;
-    const double se = f * (div_r(X.e, X.d, X.rd) + (ss - X.un) * (ss + X.p / q));
+    const double se = f * (div_r(X.e, X.d, X.rd) + (ss - X.un) * (ss + ddiv(X.p, q)));
     const Flux Fs{F.d + S * (sd - X.d), F.mn + S * (smn - X.mn), F.mt + S * (smt - X.mt),
                   F.e + S * (se - X.e)};
     return {star ? Fs.d : F.d, star ? Fs.mn : F.mn, star ? Fs.mt : F.mt, star ? Fs.e : F.e};
@@ -232,4 +271,5 @@ NPDE_HD int remap(int i, int n, int bc) {
     return i < 0 ? 0 : n - 1;
 }
 
+}  // namespace NPDE_ARITH_NS
 }  // namespace npde

@@ -22,8 +22,9 @@
 #include "npde_device.hpp"
 
 namespace npde {
-hipError_t launch_stage(int scheme, int lim, int stage, bool corr, int waves_per_block, const InstDesc* desc,
-                        Ctl* ctl, const Task* tasks, int ntasks, Partial* part, int* ndone, hipStream_t s);
+hipError_t launch_stage(int scheme, int lim, int stage, bool corr, bool fast, int waves_per_block,
+                        const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks, Partial* part, int* ndone,
+                        hipStream_t s);
 hipError_t launch_wavespeed(int waves_per_block, const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks,
                             Partial* part, int* ndone, hipStream_t s);
 hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int nftasks, int* ndone,
@@ -63,6 +64,7 @@ struct npde_solver {
     std::vector<int> traj_sizes;
     int traj_nout = -1;
     bool has_corr = false;
+    bool fast = false;  // NPDE_ARITH_FAST: the fast-arithmetic stage kernels
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     npde_tuning tune{0, 4, 32, 1};
@@ -71,6 +73,7 @@ struct npde_solver {
     int graph_key_steps = 0;
     int graph_key_corr = -1;
     int graph_key_rows = -1, graph_key_wpb = -1;
+    int graph_key_fast = -1;
     std::string err;
     double last_ms = 0.0;
 };
@@ -187,9 +190,9 @@ npde_status fill_bad(npde_solver* s, npde_bad_cell* bad) {
 // One RK2 step of every stepping instance: two stage launches, then the per-instance reduction of
 // the fused wave-speed partials and the run-state decision.
 npde_status launch_step_pair(npde_solver* s) {
-    HIPCHK(s, launch_stage(s->flux, s->lim, 1, s->has_corr, s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks,
+    HIPCHK(s, launch_stage(s->flux, s->lim, 1, s->has_corr, s->fast, s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks,
                            int(s->htasks.size()), s->dpart, s->dndone, s->stream));
-    HIPCHK(s, launch_stage(s->flux, s->lim, 2, s->has_corr, s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks,
+    HIPCHK(s, launch_stage(s->flux, s->lim, 2, s->has_corr, s->fast, s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks,
                            int(s->htasks.size()), s->dpart, s->dndone, s->stream));
     HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterStep, s->dndone, s->stream));
     return NPDE_OK;
@@ -217,7 +220,8 @@ npde_status launch_iterations(npde_solver* s, int iters, int nft) {
 npde_status ensure_graph(npde_solver* s, int nft) {
     const int K = std::max(1, s->tune.steps_per_graph);
     if (s->graph && s->graph_key_frames == nft && s->graph_key_steps == K && s->graph_key_corr == int(s->has_corr) &&
-        s->graph_key_rows == s->tune.rows_per_task && s->graph_key_wpb == s->tune.waves_per_block)
+        s->graph_key_rows == s->tune.rows_per_task && s->graph_key_wpb == s->tune.waves_per_block &&
+        s->graph_key_fast == int(s->fast))
         return NPDE_OK;
     drop_graph(s);
     hipGraph_t g = nullptr;
@@ -240,6 +244,7 @@ npde_status ensure_graph(npde_solver* s, int nft) {
     s->graph_key_corr = int(s->has_corr);
     s->graph_key_rows = s->tune.rows_per_task;
     s->graph_key_wpb = s->tune.waves_per_block;
+    s->graph_key_fast = int(s->fast);
     return NPDE_OK;
 }
 
@@ -578,10 +583,10 @@ npde_status npde_time_stages(npde_solver* s, int nsteps, double* ms_stage1, doub
     for (int k = 0; k < nsteps && st == NPDE_OK; ++k) {
         HIPCHK(s, launch_prep_host_step(s->dctl, s->n, s->ddts, s->stream));
         HIPCHK(s, hipEventRecord(e[0], s->stream));
-        HIPCHK(s, launch_stage(s->flux, s->lim, 1, s->has_corr, s->tune.waves_per_block, s->ddesc, s->dctl,
+        HIPCHK(s, launch_stage(s->flux, s->lim, 1, s->has_corr, s->fast, s->tune.waves_per_block, s->ddesc, s->dctl,
                                s->dtasks, int(s->htasks.size()), s->dpart, s->dndone, s->stream));
         HIPCHK(s, hipEventRecord(e[1], s->stream));
-        HIPCHK(s, launch_stage(s->flux, s->lim, 2, s->has_corr, s->tune.waves_per_block, s->ddesc, s->dctl,
+        HIPCHK(s, launch_stage(s->flux, s->lim, 2, s->has_corr, s->fast, s->tune.waves_per_block, s->ddesc, s->dctl,
                                s->dtasks, int(s->htasks.size()), s->dpart, s->dndone, s->stream));
         HIPCHK(s, hipEventRecord(e[2], s->stream));
         HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterStep, s->dndone, s->stream));
@@ -743,6 +748,20 @@ npde_status npde_set_tuning(npde_solver* s, const npde_tuning* t) {
     return NPDE_OK;
 }
 
+npde_status npde_set_arithmetic(npde_solver* s, int arith) {
+    if (!s) return NPDE_ERR_INVALID;
+    if (arith != NPDE_ARITH_EXACT && arith != NPDE_ARITH_FAST) return fail(s, NPDE_ERR_INVALID, "bad arithmetic mode");
+    s->fast = arith == NPDE_ARITH_FAST;
+    drop_graph(s);
+    return NPDE_OK;
+}
+
+npde_status npde_get_arithmetic(const npde_solver* s, int* arith) {
+    if (!s || !arith) return NPDE_ERR_INVALID;
+    *arith = s->fast ? NPDE_ARITH_FAST : NPDE_ARITH_EXACT;
+    return NPDE_OK;
+}
+
 npde_status npde_last_elapsed_ms(const npde_solver* s, double* ms) {
     if (!s || !ms) return NPDE_ERR_INVALID;
     *ms = s->last_ms;
@@ -751,7 +770,6 @@ npde_status npde_last_elapsed_ms(const npde_solver* s, double* ms) {
 
 void* npde_stream(npde_solver* s) { return s ? (void*)s->stream : nullptr; }
 
-// Built-in initial conditions of the driver (main2d.cpp:52-135), evaluated with the host libm.
 npde_status npde_selftest_arith(const double* x, const double* y, long long n, double* q_fast, double* q_ieee,
                                 double* root) {
     if (!x || !y || !q_fast || !q_ieee || !root || n <= 0) return NPDE_ERR_INVALID;
@@ -769,6 +787,7 @@ npde_status npde_selftest_arith(const double* x, const double* y, long long n, d
     return e == hipSuccess ? NPDE_OK : NPDE_ERR_DEVICE;
 }
 
+// Built-in initial conditions of the driver (main2d.cpp:52-135), evaluated with the host libm.
 npde_status npde_builtin_problem(const char* test, int n, npde_grid_desc* grid, double* t_end, double* U) {
     if (!test || !grid || !t_end || n <= 0) return NPDE_ERR_INVALID;
     const std::string name(test);
@@ -845,7 +864,7 @@ npde_status npde_builtin_problem(const char* test, int n, npde_grid_desc* grid, 
 npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* chosen) {
     if (!s) return NPDE_ERR_INVALID;
     std::ostringstream key;
-    key << "flux" << s->flux << "_lim" << s->lim << "_n" << s->n;
+    key << "flux" << s->flux << "_lim" << s->lim << (s->fast ? "_fast" : "") << "_n" << s->n;
     {
         std::map<std::string, int> sizes;
         for (const auto& g : s->grids) sizes[std::to_string(g.nx) + "x" + std::to_string(g.ny) + "b" +

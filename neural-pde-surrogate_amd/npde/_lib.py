@@ -65,6 +65,8 @@ _SIGNATURES = [
     ("npde_get_tuning", C.c_int, [_H, C.POINTER(Tuning)]),
     ("npde_set_tuning", C.c_int, [_H, C.POINTER(Tuning)]),
     ("npde_autotune", C.c_int, [_H, C.c_char_p, C.POINTER(Tuning)]),
+    ("npde_set_arithmetic", C.c_int, [_H, C.c_int]),
+    ("npde_get_arithmetic", C.c_int, [_H, C.POINTER(C.c_int)]),
     ("npde_last_elapsed_ms", C.c_int, [_H, _D]),
     ("npde_stream", C.c_void_p, [_H]),
     ("npde_builtin_problem", C.c_int, [C.c_char_p, C.c_int, C.POINTER(GridDesc), _D, _D]),

@@ -2,7 +2,11 @@
 
 A stack of 3x3 periodic convolutions (4 hidden layers of 32 channels with tanh-GELU, then a linear
 head; 32,392 parameters) maps the coarse state to 8 per-face flux increments: 4 conserved components
-on the face left of each cell and 4 on the face below it, each bounded by out_scale * tanh.  The increments are added to the solver's
+on the face left of each cell and 4 on the face below it.  Each increment is out_scale * tanh(raw)
+times the face's natural flux scale (rho a, rho a^2, rho a^2, rho a^3 with the smaller density and
+the larger sound speed of the two cells sharing the face), so a correction stays a bounded fraction of
+the flux it corrects, also in rarefied, low-pressure regions where a fixed-size increment would drive
+the state non-physical.  The increments are added to the solver's
 face fluxes (HLLC + van Leer) inside the stage kernels, so the update stays in flux form: mass,
 momentum and energy are conserved to rounding whatever the network outputs.  The head is zero
 initialised, so an untrained corrector reproduces the plain solver exactly (analog: the zero-init
@@ -25,21 +29,37 @@ IN_FEATURES = 8
 FEATURE_NORM = [(1.0, 1.0), (0.0, 1.0), (0.0, 1.0), (2.5, 2.5), (0.0, 1.0), (0.0, 1.0), (1.0, 1.0), (1.2, 0.6)]
 
 
-def features(U: torch.Tensor) -> torch.Tensor:
-    """(B, 4, H, W) float64 conserved -> (B, H, W, 16) float32 normalised features (channels 8-15 zero)."""
+def primitives(U: torch.Tensor):
     d, m, n, e = U[:, 0], U[:, 1], U[:, 2], U[:, 3]
     u, v = m / d, n / d
     p = (GAMMA - 1.0) * (e - 0.5 * d * (u * u + v * v))
     a = torch.sqrt(torch.clamp(GAMMA * p / d, min=0.0))
-    f = torch.stack([d, m, n, e, u, v, p, a], dim=-1)
+    return d, u, v, p, a
+
+
+def features(U: torch.Tensor) -> torch.Tensor:
+    """(B, 4, H, W) float64 conserved -> (B, H, W, 16) float32 normalised features (channels 8-15 zero)."""
+    d, u, v, p, a = primitives(U)
+    f = torch.stack([d, U[:, 1], U[:, 2], U[:, 3], u, v, p, a], dim=-1)
     off = torch.tensor([o for o, _ in FEATURE_NORM], dtype=f.dtype, device=f.device)
     sc = torch.tensor([s for _, s in FEATURE_NORM], dtype=f.dtype, device=f.device)
     f = ((f - off) / sc).to(torch.float32)
     return torch.cat([f, torch.zeros_like(f)], dim=-1).contiguous()
 
 
+def face_scales(U: torch.Tensor):
+    """Flux scales (B, 4, H, W) of the face left of each cell (x) and below it (y), periodic neighbours."""
+    d, _, _, _, a = primitives(U)
+    out = []
+    for dim in (-1, -2):
+        df = torch.minimum(d, d.roll(1, dims=dim))
+        af = torch.maximum(a, a.roll(1, dims=dim))
+        out.append(torch.stack([df * af, df * af * af, df * af * af, df * af * af * af], dim=1))
+    return out
+
+
 class FluxCorrector(torch.nn.Module):
-    def __init__(self, hidden: int = 32, hidden_layers: int = 4, out_scale: float = 0.05, seed: int = 0):
+    def __init__(self, hidden: int = 32, hidden_layers: int = 4, out_scale: float = 0.02, seed: int = 0):
         super().__init__()
         chans = [IN_FEATURES] + [hidden] * hidden_layers + [8]
         g = torch.Generator().manual_seed(seed)
@@ -66,11 +86,12 @@ class FluxCorrector(torch.nn.Module):
     def forward(self, U: torch.Tensor):
         """(B, 4, H, W) float64 state -> (corr_x, corr_y), each (B, 4, H, W) float64."""
         # the network's input is not differentiated: gradients reach it through the flux increments only
-        out = FluxCNN.apply(features(U.detach()), *self.params())  # (B, H, W, 8) float32
-        # bounded increments (|dF| < out_scale per face): a rollout cannot be driven far off by one call
+        Ud = U.detach()
+        out = FluxCNN.apply(features(Ud), *self.params())  # (B, H, W, 8) float32
         out = (torch.tanh(out) * self.out_scale).to(torch.float64)
-        cx = out[..., 0:4].permute(0, 3, 1, 2).contiguous()
-        cy = out[..., 4:8].permute(0, 3, 1, 2).contiguous()
+        sx, sy = face_scales(Ud)
+        cx = (out[..., 0:4].permute(0, 3, 1, 2) * sx).contiguous()
+        cy = (out[..., 4:8].permute(0, 3, 1, 2) * sy).contiguous()
         return cx, cy
 
     def state(self) -> dict:

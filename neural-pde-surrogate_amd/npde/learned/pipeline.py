@@ -93,42 +93,55 @@ def train(data: torch.Tensor, iters=600, batch=32, unroll=8, lr=1e-3, warmup=50,
     dx = 1.0 / n
     tf = torch.tensor(frame_times(F - 1), dtype=torch.float64, device=dev)
     scales = SCALES.to(dev).view(1, 4, 1, 1)
-    hist = []
+    hist, ratios, skipped = [], [], 0
     t0 = time.perf_counter()
-    for it in range(iters):
-        nu = unroll if curriculum is None else curriculum(it)
-        traj = torch.randint(0, N, (batch,), generator=gen)
-        k0 = torch.randint(0, F - 1 - nu + 1, (batch,), generator=gen)
+    def rollout_loss(traj, k0, nu, corrected=True):
+        """Mean over the nu intervals of the normalised L1 error; also the samples that went non-finite."""
         U = data[traj, k0].to(torch.float64)
         t = tf[k0.to(dev)]
-        loss, skip = 0.0, False
+        loss = 0.0
         for j in range(1, nu + 1):
             tn = tf[(k0 + j).to(dev)]
 
             def interval(U_, t_, tn_=tn):
-                cx, cy = model(U_)
-                U2, t2, _ = ts.advance(U_, t_, tn_, dx, dx, cx_corr=cx, cy_corr=cy)
+                if corrected:
+                    cx, cy = model(U_)
+                    U2, t2, _ = ts.advance(U_, t_, tn_, dx, dx, cx_corr=cx, cy_corr=cy)
+                else:
+                    U2, t2, _ = ts.advance(U_, t_, tn_, dx, dx)
                 return U2, t2
 
-            if recompute:
+            if recompute and corrected:
                 U, t = torch.utils.checkpoint.checkpoint(interval, U, t, use_reentrant=False)
             else:
                 U, t = interval(U, t)
             target = data[traj, k0 + j].to(torch.float64)
             err = ((U - target).abs() / scales).mean(dim=(1, 2, 3))
-            if not bool(torch.isfinite(err).all()):
-                badk = torch.nonzero(~torch.isfinite(err)).flatten().tolist()
-                log(f"  iter {it:4d}  interval {j}: non-finite state for trajectories "
-                    f"{[int(traj[b]) for b in badk][:6]} from frames {[int(k0[b]) for b in badk][:6]}")
-                skip = True
-                break
+            finite = torch.isfinite(err)
+            if not bool(finite.all()):
+                return None, torch.nonzero(~finite).flatten()
             loss = loss + err.mean()
+        return loss / nu, None
+
+    for it in range(iters):
+        nu = unroll if curriculum is None else curriculum(it)
+        traj = torch.randint(0, N, (batch,), generator=gen)
+        k0 = torch.randint(0, F - 1 - nu + 1, (batch,), generator=gen)
         opt.zero_grad(set_to_none=True)
-        # a corrected rollout that left the physical states poisons every gradient: skip the batch
-        if skip or not bool(torch.isfinite(loss)):
+        loss, badk = rollout_loss(traj, k0, nu)
+        if badk is not None:
+            # a corrected rollout left the physical states: its NaNs would poison every gradient, so
+            # the batch is recomputed without those samples
+            log(f"  iter {it:4d}  interval state non-finite for trajectories "
+                f"{[int(traj[b]) for b in badk.tolist()][:6]} from frames {[int(k0[b]) for b in badk.tolist()][:6]}")
+            keep = torch.ones(batch, dtype=torch.bool)
+            keep[badk.cpu()] = False
+            traj, k0 = traj[keep], k0[keep]
+            skipped += 1
+            loss, badk = (None, None) if len(traj) == 0 else rollout_loss(traj, k0, nu)
+        if loss is None or not bool(torch.isfinite(loss)):
             sched.step()
             continue
-        loss = loss / nu
         loss.backward()
         gnorm = torch.nn.utils.clip_grad_norm_(model.parameters(), 1.0)
         if torch.isfinite(gnorm):
@@ -136,8 +149,24 @@ def train(data: torch.Tensor, iters=600, batch=32, unroll=8, lr=1e-3, warmup=50,
         sched.step()
         hist.append(float(loss.detach()))
         if it % 25 == 0 or it == iters - 1:
-            log(f"  iter {it:4d}  loss {float(loss.detach()):.5f}  |g| {float(gnorm):.3g}  lr {sched.get_last_lr()[0]:.2e}  "
+            with torch.no_grad():
+                plain, _ = rollout_loss(traj, k0, nu, corrected=False)
+            ratio = float(loss.detach()) / float(plain) if plain is not None else float("nan")
+            log(f"  iter {it:4d}  unroll {nu}  loss {float(loss.detach()):.5f}  ratio to plain {ratio:.4f}  "
+                f"|g| {float(gnorm):.3g}  lr {sched.get_last_lr()[0]:.2e}  skipped {skipped}  "
                 f"({time.perf_counter() - t0:.0f} s)")
+            ratios.append(ratio)
+    # held-out check of the same loss: fixed (trajectory, frame) pairs, full unroll, no gradients
+    vg = torch.Generator().manual_seed(12345)
+    vt = torch.randint(0, N, (batch,), generator=vg)
+    vk = torch.randint(0, F - 1 - unroll + 1, (batch,), generator=vg)
+    with torch.no_grad():
+        lv, _ = rollout_loss(vt, vk, unroll)
+        pv, _ = rollout_loss(vt, vk, unroll, corrected=False)
+    model.val_ratio = float(lv) / float(pv) if lv is not None and pv is not None else float("nan")
+    log(f"  final loss ratio to the plain coarse solver (unroll {unroll}, {batch} fixed samples): "
+        f"{model.val_ratio:.4f}")
+    model.train_ratios = ratios
     return model, hist
 
 
@@ -314,6 +343,7 @@ def main(argv=None):
     res.update({"train_trajectories": args.train_traj, "iters": args.iters, "batch": args.batch,
                 "unroll": args.unroll, "generate_seconds": gen_s, "train_seconds": train_s,
                 "loss_first": hist[0], "loss_last": float(np.mean(hist[-20:])),
+                "train_loss_ratio_to_plain": model.val_ratio,
                 "parameters": model.num_parameters()})
     (out / "learned_eval.json").write_text(json.dumps(res, indent=1))
     log(json.dumps(res))

@@ -296,6 +296,21 @@ class Batch:
             setattr(t, k, int(v))
         check(self._h, self._lib.npde_set_tuning(self._h, C.byref(t)))
 
+    ARITH = {"exact": 0, "fast": 1}
+
+    @property
+    def arithmetic(self) -> str:
+        a = C.c_int()
+        check(self._h, self._lib.npde_get_arithmetic(self._h, C.byref(a)))
+        return "fast" if a.value == 1 else "exact"
+
+    def set_arithmetic(self, mode: str):
+        """'exact' (default; bit-identical to the reference solver) or 'fast' (reciprocal divisions,
+        shortened square root, fused multiply-adds: rounding-level differences per step)."""
+        if mode not in self.ARITH:
+            raise ValueError(f"arithmetic must be 'exact' or 'fast', not {mode!r}")
+        check(self._h, self._lib.npde_set_arithmetic(self._h, self.ARITH[mode]))
+
     def autotune(self, cache_path: str | None = None) -> dict:
         t = Tuning()
         check(self._h, self._lib.npde_autotune(self._h, cache_path.encode() if cache_path else None, C.byref(t)))

@@ -57,6 +57,21 @@ def npde_mod():
     return npde
 
 
+def bitwise_equal(a, b) -> bool:
+    """Same shape, dtype and bit pattern element by element (so +0 != -0); NaNs match NaNs at the same
+    positions whatever their payload (failed states hold NaNs)."""
+    a, b = np.asarray(a), np.asarray(b)
+    if a.shape != b.shape or a.dtype != b.dtype:
+        return False
+    if a.dtype.kind != "f":
+        return np.array_equal(a, b)
+    na, nb = np.isnan(a), np.isnan(b)
+    if not np.array_equal(na, nb):
+        return False
+    u = {8: np.uint64, 4: np.uint32, 2: np.uint16}[a.itemsize]
+    return np.array_equal(a.view(u)[~na], b.view(u)[~nb])
+
+
 def run_case_grid(arr):
     """Grid from a fixture's (nx, ny, x_min, x_max, y_min, y_max, bc_x, bc_y) vector."""
     import oracle2d

@@ -1,12 +1,12 @@
 """GPU parity: the HIP path (libnpde.so through its C-ABI) against the oracle and the golden vectors.
 
-Bar: bit-for-bit equality (IEEE ==) of every conserved value, every dt and the step count -- the
+Bar: bit-for-bit equality (bit patterns, so +0 and -0 differ) of every conserved value, every dt and the step count -- the
 reference solver is binary64 without contraction, and so are the kernels.
 """
 import numpy as np
 import pytest
 
-from conftest import run_case_grid
+from conftest import bitwise_equal, run_case_grid
 
 import oracle2d
 
@@ -20,7 +20,7 @@ RUN_NAMES = ["lax_liu3_64_hllc_vanleer", "lax_liu3_48_rusanov_minmod", "sod_x_64
 
 
 def same(a, b):
-    return np.array_equal(np.asarray(a), np.asarray(b), equal_nan=True)
+    return bitwise_equal(a, b)
 
 
 def to_npde_grid(npde, g):
@@ -230,3 +230,76 @@ def test_full_size_batch_properties(npde_mod, oracle):
     for j, k in enumerate(range(0, B, 9)):
         m0 = ics[k][0].sum()
         assert abs(mass[j] - m0) <= 1e-12 * abs(m0)
+
+
+def test_uneven_blast_batch_full_size(npde_mod, oracle):
+    """The uneven workload: 96 blast/bubble instances of 128^2-1024^2 (weights 4:3:2:1), periodic, in
+    one batch for a few steps.  Each instance of every size equals its own single-instance run bit for
+    bit, a 128^2 instance equals the oracle, and periodic totals are conserved to rounding."""
+    from npde.families import blast, blast_size
+    B, STEPS = 96, 6
+    sizes = [blast_size(k) for k in range(B)]
+    ics = [blast(k)[0] for k in range(B)]
+    grids = [npde_mod.make_grid(n, n, bc_x=1, bc_y=1) for n in sizes]
+    with npde_mod.Batch(grids, "hllc", "vanleer", 0.5) as b:
+        for k in range(B):
+            b.set_state(k, ics[k])
+        b.advance_steps(STEPS, 0.2)
+        firsts = [sizes.index(n) for n in (128, 256, 512, 1024)]
+        got = {k: b.get_state(k) for k in firsts}
+        assert all(b.time(k)[1] == STEPS for k in range(B))
+        mass = [(b.get_state(k)[0].sum(), ics[k][0].sum()) for k in range(0, B, 7)]
+    for k in firsts:
+        with npde_mod.Batch([grids[k]], "hllc", "vanleer", 0.5) as s:
+            s.set_state(0, ics[k])
+            s.advance_steps(STEPS, 0.2)
+            assert same(s.get_state(0), got[k]), sizes[k]
+    k = firsts[0]
+    go = oracle2d.Grid.make(128, 128, bc_x="periodic", bc_y="periodic")
+    a = ics[k].copy()
+    ro = oracle.run(go, "hllc", "vanleer", 0.5, a, 0.2, max_steps=STEPS)
+    assert ro["steps"] == STEPS and same(got[k], a)
+    for m1, m0 in mass:
+        assert abs(m1 - m0) <= 1e-12 * abs(m0)
+
+
+FAST_TOL_10_STEPS = 1e-12  # max |fast - exact| / max|exact| per variable after 10 steps (measured ~5e-16)
+
+
+def test_fast_arithmetic_is_opt_in_and_close_to_exact(npde_mod, oracle):
+    """NPDE_ARITH_FAST (reciprocal divisions, shortened square root, contraction) stays within
+    rounding-level distance of the exact kernels over 10 Lax-Liu steps, and switching back to
+    NPDE_ARITH_EXACT reproduces the reference bits."""
+    g, t_end, U0 = npde_mod.builtin_problem("lax_liu3", 96)
+    go, _, _ = oracle.setup("lax_liu3", 96)
+    with npde_mod.Batch([g], "hllc", "vanleer", 0.5) as b:
+        assert b.arithmetic == "exact"
+        b.set_arithmetic("fast")
+        assert b.arithmetic == "fast"
+        b.set_state(0, U0)
+        b.advance_steps(10, t_end)
+        fast = b.get_state(0)
+        b.set_arithmetic("exact")
+        b.set_state(0, U0)
+        b.advance_steps(10, t_end)
+        exact = b.get_state(0)
+    a = U0.copy()
+    oracle.run(go, "hllc", "vanleer", 0.5, a, t_end, max_steps=10)
+    assert same(exact, a)
+    scale = np.abs(exact).reshape(4, -1).max(axis=1).reshape(4, 1, 1)
+    dev = float(np.max(np.abs(fast - exact) / scale))
+    assert 0.0 < dev <= FAST_TOL_10_STEPS, dev
+
+
+def test_fast_arithmetic_conserves_periodic_totals(npde_mod):
+    """The fast kernels keep the flux form: periodic totals of all four conserved variables are
+    conserved to rounding."""
+    from npde.families import riemann
+    U0 = riemann(3, 128)[0]
+    with npde_mod.Batch([npde_mod.make_grid(128, 128, bc_x=1, bc_y=1)], "hllc", "vanleer", 0.5) as b:
+        b.set_arithmetic("fast")
+        b.set_state(0, U0)
+        b.advance_steps(40, 0.5)
+        U = b.get_state(0)
+    for q in range(4):
+        assert abs(U[q].sum() - U0[q].sum()) <= 1e-12 * np.abs(U0[q]).sum(), q

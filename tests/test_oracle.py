@@ -13,14 +13,15 @@ import pytest
 from conftest import run_case_grid
 
 import oracle2d
+from conftest import bitwise_equal
 
 SCHEMES = ["rusanov", "hllc"]
 LIMS = ["none", "unlimited", "minmod", "vanleer"]
 
 
 def same(a, b):
-    """IEEE equality element by element (+0 == -0); NaN matches NaN (failed states hold NaNs)"""
-    return np.array_equal(np.asarray(a), np.asarray(b), equal_nan=True)
+    """Bit-for-bit equality (+0 and -0 differ); NaN matches NaN (failed states hold NaNs)"""
+    return bitwise_equal(a, b)
 
 
 # ---------------------------------------------------------------- 1. golden vectors
