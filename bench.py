@@ -190,7 +190,7 @@ def main():
         "config": {"workload": "random-Riemann batch: 64 instances x 512^2, periodic, HLLC + van Leer MUSCL, "
                                "SSP-RK2, CFL 0.5, per-instance dt, device-resident loop (hipGraph)",
                    "instances": N_INST, "n": N, "tuning": tuning},
-        "roofline": {"bound": "hbm", "kernel": "k_stage<HLLC, vanLeer, stage 2>", "achieved": ach2,
+        "roofline": {"bound": "hbm", "kernel": "npde::exact::k_stage<HLLC, vanLeer, 2, false>", "achieved": ach2,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach2 / HBM_PEAK_GBS, "traffic": traffic(),
                      "bytes_per_cell": BYTES_STAGE2, "launch_ms": ms2, "stage1_launch_ms": ms1,
                      "stage1_achieved": ach1},

@@ -82,6 +82,29 @@ def conv3x3_wgrad(x, g):
     return dw.view(3, 3, cin, cout).permute(3, 2, 0, 1), db
 
 
+def pack_stack(params):
+    """Per layer (packed BF16 weights, zero-padded float32 bias, padded Cout, stored Cout) for inference."""
+    ws, bs = params[0::2], params[1::2]
+    out = []
+    for k, (w, b) in enumerate(zip(ws, bs)):
+        cout = w.shape[0]
+        cout_p = _pad16(cout)
+        bias = torch.zeros(cout_p, device=w.device, dtype=torch.float32)
+        bias[:cout] = b.detach()
+        last = k == len(ws) - 1
+        out.append((pack_weights(w.detach()), bias, cout_p, cout if last else cout_p))
+    return out
+
+
+def flux_cnn_infer(x, packed):
+    """Forward of the stack without the pre-activations the backward needs (inference only)."""
+    h = x
+    for k, (wp, bias, cout_p, cout_store) in enumerate(packed):
+        last = k == len(packed) - 1
+        h, _ = conv3x3(h, wp, bias, cout_p, cout_store, EPI_LINEAR if last else EPI_GELU)
+    return h
+
+
 class FluxCNN(torch.autograd.Function):
     """Whole network as one autograd node; see the module docstring."""
 

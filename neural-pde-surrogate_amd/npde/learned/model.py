@@ -21,7 +21,7 @@ import math
 
 import torch
 
-from .conv import FluxCNN
+from .conv import FluxCNN, flux_cnn_infer, pack_stack
 
 GAMMA = 1.4
 IN_FEATURES = 8
@@ -94,6 +94,19 @@ class FluxCorrector(torch.nn.Module):
         cy = (out[..., 4:8].permute(0, 3, 1, 2) * sy).contiguous()
         return cx, cy
 
+    def increments_into(self, U: torch.Tensor, CX: torch.Tensor, CY: torch.Tensor, keep: torch.Tensor, packed):
+        """Inference: the increments of forward(U) written into CX, CY (B, 4, H, W) float64, zero for
+        instances whose keep flag is False.  `packed` = pack_stack(self.params()), computed once; no
+        tensor is allocated outside torch's allocator and nothing synchronises the host, so the call
+        can be captured in a graph."""
+        out = flux_cnn_infer(features(U), packed)
+        out = (torch.tanh(out) * self.out_scale).to(torch.float64)
+        sx, sy = face_scales(U)
+        k = keep.view(-1, 1, 1, 1)
+        zero = torch.zeros((), dtype=torch.float64, device=U.device)
+        CX.copy_(torch.where(k, out[..., 0:4].permute(0, 3, 1, 2) * sx, zero))
+        CY.copy_(torch.where(k, out[..., 4:8].permute(0, 3, 1, 2) * sy, zero))
+
     def state(self) -> dict:
         return {f"w{k}": w.detach().cpu() for k, w in enumerate(self.weights)} | \
                {f"b{k}": b.detach().cpu() for k, b in enumerate(self.biases)} | \
@@ -104,3 +117,16 @@ class FluxCorrector(torch.nn.Module):
             self.weights[k].data.copy_(torch.as_tensor(st[f"w{k}"]))
             self.biases[k].data.copy_(torch.as_tensor(st[f"b{k}"]))
         self.out_scale = float(st["out_scale"])
+
+
+TRAINED_WEIGHTS = __import__("pathlib").Path(__file__).resolve().parent / "weights" / "flux_corrector_riemann64.npz"
+
+
+def load_trained(path=TRAINED_WEIGHTS, device="cuda") -> FluxCorrector:
+    """The corrector trained on the random-Riemann family at 64^2 (weights stored as a plain .npz of
+    float32 arrays, read without pickle)."""
+    import numpy as np
+    st = dict(np.load(path, allow_pickle=False))
+    m = FluxCorrector()
+    m.load({k: torch.from_numpy(v) for k, v in st.items()})
+    return m.to(device)

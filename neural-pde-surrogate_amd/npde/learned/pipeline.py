@@ -38,6 +38,7 @@ from .. import solver as S
 from .._lib import NonPhysicalStateError
 from ..families import riemann
 from . import torch_solver as ts
+from .conv import pack_stack
 from .model import FluxCorrector
 
 FINE, COARSE = 512, 64
@@ -202,11 +203,13 @@ def learned_rollout(model: FluxCorrector, indices, n=COARSE, batch=64, force_fal
     and the indices that fell back to the plain fine solve (whose frames replace theirs).
 
     The network reads the solver's state in place (zero-copy view) and writes its increments into
-    fixed buffers the stage kernels read; per frame interval the host does one network call, one
-    device loop to the frame time and one positivity/indicator check."""
+    fixed buffers the stage kernels read.  Everything runs on the solver's stream: per frame interval,
+    one replay of the captured network graph, one device loop to the frame time, the frame copy and
+    the positivity / indicator check, with no host synchronisation besides the loop's own."""
     dev = torch.device("cuda")
     tf = frame_times()
     frames_all, secs, fell = [], 0.0, []
+    packed = pack_stack(model.params())
     for s0 in range(0, len(indices), batch):
         idx = list(indices[s0:s0 + batch])
         B = len(idx)
@@ -216,41 +219,46 @@ def learned_rollout(model: FluxCorrector, indices, n=COARSE, batch=64, force_fal
             b.set_tuning(steps_per_graph=8)  # ~14 CFL substeps per frame interval at 64^2
             for k in range(B):
                 b.set_state(k, U0[k])
-            U = b.state_view()
-            rho_max = U[:, 0].flatten(1).amax(1)
-            CX = torch.zeros(B, 4, n, n, dtype=torch.float64, device=dev)
-            CY = torch.zeros_like(CX)
-            for k in range(B):
-                b.set_flux_correction(k, CX[k].data_ptr(), CY[k].data_ptr())
-            frames = torch.empty(B, NOUT + 1, 4, n, n, dtype=torch.float32, device=dev)
-            frames[:, 0] = U.to(torch.float32)
-            bad = torch.zeros(B, dtype=torch.bool, device=dev)
-            b.advance_to(0.0)  # warm-up: captures the graph outside the timing
-            model(U)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for j in range(1, NOUT + 1):
-                cx, cy = model(U)
-                keep = (~bad).view(-1, 1, 1, 1)
-                CX.copy_(torch.where(keep, cx, torch.zeros_like(cx)))
-                CY.copy_(torch.where(keep, cy, torch.zeros_like(cy)))
+            stream = torch.cuda.ExternalStream(b.stream)
+            with torch.cuda.stream(stream):
+                U = b.state_view()
+                rho_max = U[:, 0].flatten(1).amax(1)
+                CX = torch.zeros(B, 4, n, n, dtype=torch.float64, device=dev)
+                CY = torch.zeros_like(CX)
+                for k in range(B):
+                    b.set_flux_correction(k, CX[k].data_ptr(), CY[k].data_ptr())
+                frames = torch.empty(B, NOUT + 1, 4, n, n, dtype=torch.float32, device=dev)
+                frames[:, 0] = U.to(torch.float32)
+                keep = torch.ones(B, dtype=torch.bool, device=dev)
+                model.increments_into(U, CX, CY, keep, packed)  # warm-up (allocations, kernel loads)
+                b.advance_to(0.0)  # warm-up: captures the solver's graph outside the timing
+                net = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(net):
+                    model.increments_into(U, CX, CY, keep, packed)
                 torch.cuda.synchronize()
-                try:
-                    b.advance_to(tf[j])
-                except NonPhysicalStateError as e:
-                    bad[e.instance] = True  # its remaining frames come from the fine solve
-                frames[:, j] = U.to(torch.float32)
-                rho = U[:, 0]
-                p = (1.4 - 1.0) * (U[:, 3] - 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2) / U[:, 0])
-                ok = (torch.isfinite(U).flatten(1).all(1) & (rho.flatten(1).amin(1) > 0) & (p.flatten(1).amin(1) > 0)
-                      & (rho.flatten(1).amax(1) <= BLOWUP * rho_max))
-                bad |= ~ok
-                if force_fallback:
-                    bad[:] = True
-            torch.cuda.synchronize()
-            secs += time.perf_counter() - t0
-            out = frames.cpu().numpy()
-            badn = bad.cpu().numpy()
+                t0 = time.perf_counter()
+                for j in range(1, NOUT + 1):
+                    net.replay()
+                    try:
+                        b.advance_to(tf[j])
+                    except NonPhysicalStateError:
+                        # every instance that stopped short of the frame time met a non-physical state;
+                        # their remaining frames come from the fine solve
+                        for k in range(B):
+                            if b.time(k)[0] != tf[j]:
+                                keep[k] = False
+                    frames[:, j] = U.to(torch.float32)
+                    rho = U[:, 0]
+                    p = (1.4 - 1.0) * (U[:, 3] - 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2) / U[:, 0])
+                    ok = (torch.isfinite(U).flatten(1).all(1) & (rho.flatten(1).amin(1) > 0)
+                          & (p.flatten(1).amin(1) > 0) & (rho.flatten(1).amax(1) <= BLOWUP * rho_max))
+                    keep &= ok
+                    if force_fallback:
+                        keep.zero_()
+                torch.cuda.synchronize()
+                secs += time.perf_counter() - t0
+                out = frames.cpu().numpy()
+                badn = (~keep).cpu().numpy()
         if badn.any():
             redo = [idx[k] for k in np.nonzero(badn)[0]]
             t1 = time.perf_counter()
