@@ -40,10 +40,9 @@ def primitives(U: torch.Tensor):
 def features(U: torch.Tensor) -> torch.Tensor:
     """(B, 4, H, W) float64 conserved -> (B, H, W, 16) float32 normalised features (channels 8-15 zero)."""
     d, u, v, p, a = primitives(U)
-    f = torch.stack([d, U[:, 1], U[:, 2], U[:, 3], u, v, p, a], dim=-1)
-    off = torch.tensor([o for o, _ in FEATURE_NORM], dtype=f.dtype, device=f.device)
-    sc = torch.tensor([s for _, s in FEATURE_NORM], dtype=f.dtype, device=f.device)
-    f = ((f - off) / sc).to(torch.float32)
+    # scalar constants only (no host-to-device copies), so the function can be graph-captured
+    vals = [d, U[:, 1], U[:, 2], U[:, 3], u, v, p, a]
+    f = torch.stack([(x - o) / s for x, (o, s) in zip(vals, FEATURE_NORM)], dim=-1).to(torch.float32)
     return torch.cat([f, torch.zeros_like(f)], dim=-1).contiguous()
 
 
