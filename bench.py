@@ -124,6 +124,8 @@ def main():
     ap.add_argument("--no-fast", action="store_true", help="skip the fast-arithmetic (not bit-exact) variant")
     ap.add_argument("--rows", type=int, default=None, help="fix the band height (skips autotuning)")
     ap.add_argument("--wpb", type=int, default=None, help="fix the waves per block (skips autotuning)")
+    ap.add_argument("--row-groups", type=int, default=None, choices=(0, 1),
+                    help="fix the row-group mode (skips autotuning)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 1 GiB device copy (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -151,9 +153,10 @@ def main():
     b = npde.Batch(grids, "hllc", "vanleer", 0.5)
     for k, U in enumerate(ics):
         b.set_state(k, U)
-    if args.rows is not None or args.wpb is not None:
-        b.set_tuning(**{k: v for k, v in (("rows_per_task", args.rows), ("waves_per_block", args.wpb))
-                        if v is not None})
+    fixed = {k: v for k, v in (("rows_per_task", args.rows), ("waves_per_block", args.wpb),
+                                ("row_groups", args.row_groups)) if v is not None}
+    if fixed:
+        b.set_tuning(**fixed)
         tuning = b.tuning
     else:
         tuning = b.tuning if args.no_tune else b.autotune(None)

@@ -62,6 +62,9 @@ typedef struct {
     int waves_per_block;  /* wavefronts per workgroup: 1, 2, 4 (or 8 within the 256-lane bound) */
     int steps_per_graph;  /* RK2 steps captured per hipGraph launch in npde_run / npde_advance_to */
     int use_graph;        /* 0: plain stream launches */
+    int row_groups;       /* 1: when every instance is periodic in x with the same width 64*W (W <= 8), one
+                             workgroup of W wavefronts per band exchanges edge columns through LDS instead
+                             of recomputing a halo; 0: 60-column strips with a 2-column halo */
 } npde_tuning;
 
 typedef struct npde_solver npde_solver;

@@ -16,8 +16,9 @@
 
 namespace npde {
 
-hipError_t launch_stage_fast(int scheme, int lim, int stage, bool corr, int waves_per_block, const InstDesc* desc,
-                             Ctl* ctl, const Task* tasks, int ntasks, Partial* part, int* ndone, hipStream_t s);
+hipError_t launch_stage_fast(int scheme, int lim, int stage, bool corr, int waves_per_block, int row_waves,
+                             const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks, Partial* part, int* ndone,
+                             hipStream_t s);
 
 // ---------------------------------------------------------------------------------------------
 // Run-state machine of one instance (executed by one lane).  Mirrors the driver loop
@@ -120,22 +121,29 @@ __global__ __launch_bounds__(64) void k_finish(const InstDesc* __restrict__ desc
     finish_instance(D, c, part, lane, ndone);
 }
 
-// Max wave speeds and first non-physical cell of U for every instance that is not done.
-__global__ __launch_bounds__(256) void k_wavespeed(const InstDesc* __restrict__ desc, Ctl* __restrict__ ctl,
-                                                   const Task* __restrict__ tasks, int ntasks,
-                                                   Partial* __restrict__ part, int* __restrict__ ndone) {
+// Max wave speeds and first non-physical cell of U for every instance that is not done.  Same task
+// list, column mapping and partial slots as the stage kernels (strips or row groups).
+template <bool ROWS>
+__global__ __launch_bounds__(kMaxRowWaves * kWave) void k_wavespeed(const InstDesc* __restrict__ desc,
+                                                                    Ctl* __restrict__ ctl,
+                                                                    const Task* __restrict__ tasks, int ntasks,
+                                                                    Partial* __restrict__ part,
+                                                                    int* __restrict__ ndone) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int tid = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
-    if (tid >= ntasks) return;
-    const Task tk = tasks[tid];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int nw = blockDim.x / kWave;
+    const int task = ROWS ? int(blockIdx.x) : __builtin_amdgcn_readfirstlane(blockIdx.x * nw + wv);
+    const int tid = ROWS ? __builtin_amdgcn_readfirstlane(blockIdx.x * nw + wv) : task;
+    if (task >= ntasks) return;
+    const Task tk = tasks[task];
     Ctl* c = ctl + tk.inst;
     if (c->done) return;
     const InstDesc& D = desc[tk.inst];
     const int nx = D.nx;
     const long long P = D.plane;
-    const int i = tk.i0 - kHalo + lane;
-    const int ci = remap(i, nx, D.bc_x);
-    const bool owner = lane >= kHalo && lane < kWave - kHalo && i < nx;
+    const int i = ROWS ? wv * kWave + lane : tk.i0 - kHalo + lane;
+    const int ci = ROWS ? i : remap(i, nx, D.bc_x);
+    const bool owner = ROWS || (lane >= kHalo && lane < kWave - kHalo && i < nx);
     double sx = 0.0, sy = 0.0;
     int bad = kNoBad;
     const int ny = D.ny;
@@ -256,22 +264,41 @@ hipError_t launch_selftest_arith(const double* x, const double* y, long long n, 
     return hipGetLastError();
 }
 
-hipError_t launch_stage(int scheme, int lim, int stage, bool corr, bool fast, int waves_per_block,
+// Launch shape: strips put waves_per_block tasks (one per wavefront) in a workgroup; row groups
+// (row_waves = W > 0) give each task (band) its own workgroup of W wavefronts.
+static inline void launch_dims(int ntasks, int waves_per_block, int row_waves, dim3& grid, dim3& block) {
+    if (row_waves > 0) {
+        grid = dim3(unsigned(ntasks));
+        block = dim3(unsigned(row_waves * kWave));
+    } else {
+        grid = dim3(unsigned((ntasks + waves_per_block - 1) / waves_per_block));
+        block = dim3(unsigned(waves_per_block * kWave));
+    }
+}
+
+hipError_t launch_stage(int scheme, int lim, int stage, bool corr, bool fast, int waves_per_block, int row_waves,
                         const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks, Partial* part, int* ndone,
                         hipStream_t s) {
-    if (fast) return launch_stage_fast(scheme, lim, stage, corr, waves_per_block, desc, ctl, tasks, ntasks, part,
-                                       ndone, s);
-    const int blocks = (ntasks + waves_per_block - 1) / waves_per_block;
-    hipLaunchKernelGGL(pick_stage(scheme, lim, stage, corr), dim3(blocks), dim3(waves_per_block * kWave), 0, s,
-                       desc, ctl, tasks, ntasks, part, ndone);
+    if (row_waves < 0 || row_waves > kMaxRowWaves) return hipErrorInvalidValue;
+    if (fast)
+        return launch_stage_fast(scheme, lim, stage, corr, waves_per_block, row_waves, desc, ctl, tasks, ntasks, part,
+                                 ndone, s);
+    dim3 grid, block;
+    launch_dims(ntasks, waves_per_block, row_waves, grid, block);
+    hipLaunchKernelGGL(pick_stage(scheme, lim, stage, corr, row_waves), grid, block, 0, s, desc, ctl, tasks, ntasks,
+                       part, ndone);
     return hipGetLastError();
 }
 
-hipError_t launch_wavespeed(int waves_per_block, const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks,
-                            Partial* part, int* ndone, hipStream_t s) {
-    const int blocks = (ntasks + waves_per_block - 1) / waves_per_block;
-    hipLaunchKernelGGL(k_wavespeed, dim3(blocks), dim3(waves_per_block * kWave), 0, s, desc, ctl, tasks, ntasks,
-                       part, ndone);
+hipError_t launch_wavespeed(int waves_per_block, int row_waves, const InstDesc* desc, Ctl* ctl, const Task* tasks,
+                            int ntasks, Partial* part, int* ndone, hipStream_t s) {
+    if (row_waves < 0 || row_waves > kMaxRowWaves) return hipErrorInvalidValue;
+    dim3 grid, block;
+    launch_dims(ntasks, waves_per_block, row_waves, grid, block);
+    if (row_waves > 0)
+        hipLaunchKernelGGL(k_wavespeed<true>, grid, block, 0, s, desc, ctl, tasks, ntasks, part, ndone);
+    else
+        hipLaunchKernelGGL(k_wavespeed<false>, grid, block, 0, s, desc, ctl, tasks, ntasks, part, ndone);
     return hipGetLastError();
 }
 

@@ -22,11 +22,11 @@
 #include "npde_device.hpp"
 
 namespace npde {
-hipError_t launch_stage(int scheme, int lim, int stage, bool corr, bool fast, int waves_per_block,
+hipError_t launch_stage(int scheme, int lim, int stage, bool corr, bool fast, int waves_per_block, int row_waves,
                         const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks, Partial* part, int* ndone,
                         hipStream_t s);
-hipError_t launch_wavespeed(int waves_per_block, const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks,
-                            Partial* part, int* ndone, hipStream_t s);
+hipError_t launch_wavespeed(int waves_per_block, int row_waves, const InstDesc* desc, Ctl* ctl, const Task* tasks,
+                            int ntasks, Partial* part, int* ndone, hipStream_t s);
 hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int nftasks, int* ndone,
                         hipStream_t s);
 hipError_t launch_block_average(const InstDesc* desc, int inst, int m, int nyc, float* out, hipStream_t s);
@@ -67,7 +67,8 @@ struct npde_solver {
     bool fast = false;  // NPDE_ARITH_FAST: the fast-arithmetic stage kernels
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    npde_tuning tune{0, 4, 32, 1};
+    npde_tuning tune{0, 4, 32, 1, 1};
+    int row_waves = 0;  // > 0: the task list is in row-group form with this many wavefronts per band
     hipGraphExec_t graph = nullptr;
     int graph_key_frames = -1;  // nftasks the cached graph was captured with (0: no frame kernel)
     int graph_key_steps = 0;
@@ -120,19 +121,44 @@ int band_rows(const npde_solver* s) {
     return r;
 }
 
+// Row groups apply when every instance is periodic in x with the same width 64*W, W <= 8: then one
+// workgroup of W wavefronts covers a band and no halo column is recomputed.
+int row_group_waves_for(const npde_solver* s, const npde_tuning& tune) {
+    if (!tune.row_groups || s->n == 0) return 0;
+    const int nx = s->grids[0].nx;
+    if (nx % kWave != 0 || nx / kWave > kMaxRowWaves) return 0;
+    for (const auto& g : s->grids)
+        if (g.nx != nx || g.bc_x != NPDE_BC_PERIODIC) return 0;
+    return nx / kWave;
+}
+int row_group_waves(const npde_solver* s) { return row_group_waves_for(s, s->tune); }
+
+// Task list and partial slots.  Strips: one task and one partial per wavefront.  Row groups: one task
+// per band (workgroup), W partials per task; D.task0 / D.ntasks index partial slots in both forms.
 void build_tasks(npde_solver* s) {
     s->htasks.clear();
+    s->row_waves = row_group_waves(s);
+    const int W = s->row_waves;
     const int rows = band_rows(s);
     for (int k = 0; k < s->n; ++k) {
         InstDesc& D = s->hdesc[k];
-        D.task0 = int(s->htasks.size());
-        const int strips = (D.nx + kStripCols - 1) / kStripCols;
-        for (int j0 = 0; j0 < D.ny; j0 += rows)
-            for (int st = 0; st < strips; ++st)
-                s->htasks.push_back({k, st * kStripCols, j0, std::min(j0 + rows, D.ny)});
-        D.ntasks = int(s->htasks.size()) - D.task0;
+        const int first = int(s->htasks.size());
+        if (W > 0) {
+            for (int j0 = 0; j0 < D.ny; j0 += rows) s->htasks.push_back({k, 0, j0, std::min(j0 + rows, D.ny)});
+            D.task0 = first * W;
+            D.ntasks = (int(s->htasks.size()) - first) * W;
+        } else {
+            const int strips = (D.nx + kStripCols - 1) / kStripCols;
+            for (int j0 = 0; j0 < D.ny; j0 += rows)
+                for (int st = 0; st < strips; ++st)
+                    s->htasks.push_back({k, st * kStripCols, j0, std::min(j0 + rows, D.ny)});
+            D.task0 = first;
+            D.ntasks = int(s->htasks.size()) - first;
+        }
     }
 }
+
+size_t partial_slots(const npde_solver* s) { return s->htasks.size() * size_t(s->row_waves > 0 ? s->row_waves : 1); }
 
 npde_status upload_tasks(npde_solver* s) {
     build_tasks(s);
@@ -141,7 +167,7 @@ npde_status upload_tasks(npde_solver* s) {
     s->dtasks = nullptr;
     s->dpart = nullptr;
     HIPCHK(s, hipMalloc(&s->dtasks, sizeof(Task) * s->htasks.size()));
-    HIPCHK(s, hipMalloc(&s->dpart, sizeof(Partial) * s->htasks.size()));
+    HIPCHK(s, hipMalloc(&s->dpart, sizeof(Partial) * partial_slots(s)));
     HIPCHK(s, hipMemcpy(s->dtasks, s->htasks.data(), sizeof(Task) * s->htasks.size(), hipMemcpyHostToDevice));
     HIPCHK(s, hipMemcpy(s->ddesc, s->hdesc.data(), sizeof(InstDesc) * s->n, hipMemcpyHostToDevice));
     return NPDE_OK;
@@ -190,16 +216,16 @@ npde_status fill_bad(npde_solver* s, npde_bad_cell* bad) {
 // One RK2 step of every stepping instance: two stage launches, then the per-instance reduction of
 // the fused wave-speed partials and the run-state decision.
 npde_status launch_step_pair(npde_solver* s) {
-    HIPCHK(s, launch_stage(s->flux, s->lim, 1, s->has_corr, s->fast, s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks,
+    HIPCHK(s, launch_stage(s->flux, s->lim, 1, s->has_corr, s->fast, s->tune.waves_per_block, s->row_waves, s->ddesc, s->dctl, s->dtasks,
                            int(s->htasks.size()), s->dpart, s->dndone, s->stream));
-    HIPCHK(s, launch_stage(s->flux, s->lim, 2, s->has_corr, s->fast, s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks,
+    HIPCHK(s, launch_stage(s->flux, s->lim, 2, s->has_corr, s->fast, s->tune.waves_per_block, s->row_waves, s->ddesc, s->dctl, s->dtasks,
                            int(s->htasks.size()), s->dpart, s->dndone, s->stream));
     HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterStep, s->dndone, s->stream));
     return NPDE_OK;
 }
 
 npde_status launch_wavespeed_and_finish(npde_solver* s) {
-    HIPCHK(s, launch_wavespeed(s->tune.waves_per_block, s->ddesc, s->dctl, s->dtasks, int(s->htasks.size()),
+    HIPCHK(s, launch_wavespeed(s->tune.waves_per_block, s->row_waves, s->ddesc, s->dctl, s->dtasks, int(s->htasks.size()),
                                s->dpart, s->dndone, s->stream));
     HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterWavespeed, s->dndone, s->stream));
     return NPDE_OK;
@@ -583,10 +609,10 @@ npde_status npde_time_stages(npde_solver* s, int nsteps, double* ms_stage1, doub
     for (int k = 0; k < nsteps && st == NPDE_OK; ++k) {
         HIPCHK(s, launch_prep_host_step(s->dctl, s->n, s->ddts, s->stream));
         HIPCHK(s, hipEventRecord(e[0], s->stream));
-        HIPCHK(s, launch_stage(s->flux, s->lim, 1, s->has_corr, s->fast, s->tune.waves_per_block, s->ddesc, s->dctl,
+        HIPCHK(s, launch_stage(s->flux, s->lim, 1, s->has_corr, s->fast, s->tune.waves_per_block, s->row_waves, s->ddesc, s->dctl,
                                s->dtasks, int(s->htasks.size()), s->dpart, s->dndone, s->stream));
         HIPCHK(s, hipEventRecord(e[1], s->stream));
-        HIPCHK(s, launch_stage(s->flux, s->lim, 2, s->has_corr, s->fast, s->tune.waves_per_block, s->ddesc, s->dctl,
+        HIPCHK(s, launch_stage(s->flux, s->lim, 2, s->has_corr, s->fast, s->tune.waves_per_block, s->row_waves, s->ddesc, s->dctl,
                                s->dtasks, int(s->htasks.size()), s->dpart, s->dndone, s->stream));
         HIPCHK(s, hipEventRecord(e[2], s->stream));
         HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterStep, s->dndone, s->stream));
@@ -741,10 +767,10 @@ npde_status npde_set_tuning(npde_solver* s, const npde_tuning* t) {
     if (t->rows_per_task < 0 || !(t->waves_per_block == 1 || t->waves_per_block == 2 || t->waves_per_block == 4) ||
         t->steps_per_graph < 1)
         return fail(s, NPDE_ERR_INVALID, "bad tuning");
-    const bool rows_changed = t->rows_per_task != s->tune.rows_per_task;
+    const bool layout_changed = t->rows_per_task != s->tune.rows_per_task || t->row_groups != s->tune.row_groups;
     s->tune = *t;
     drop_graph(s);
-    if (rows_changed) return upload_tasks(s);
+    if (layout_changed) return upload_tasks(s);
     return NPDE_OK;
 }
 
@@ -873,14 +899,16 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
     }
     if (cache_path) {
         std::ifstream in(cache_path);
-        std::string k;
-        npde_tuning t = s->tune;
-        while (in >> k >> t.rows_per_task >> t.waves_per_block) {
-            if (k == key.str()) {
-                npde_status st = npde_set_tuning(s, &t);
-                if (st == NPDE_OK && chosen) *chosen = s->tune;
-                return st;
-            }
+        std::string line;
+        while (std::getline(in, line)) {  // "key rows waves_per_block [row_groups]"
+            std::istringstream ls(line);
+            std::string k;
+            npde_tuning t = s->tune;
+            if (!(ls >> k >> t.rows_per_task >> t.waves_per_block) || k != key.str()) continue;
+            if (!(ls >> t.row_groups)) t.row_groups = 0;
+            npde_status st = npde_set_tuning(s, &t);
+            if (st == NPDE_OK && chosen) *chosen = s->tune;
+            return st;
         }
     }
     // save state and run state
@@ -909,13 +937,19 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
     for (const auto& g : s->grids) max_ny = std::max(max_ny, g.ny);
     const int rows_c[] = {4, 8, 16, 32, 64, 128};
     const int wpb_c[] = {1, 2, 4};
+    npde_tuning probe = orig;
+    probe.row_groups = 1;
+    const bool rows_ok = row_group_waves_for(s, probe) > 0;  // row groups apply to this batch
+    for (int rg = rows_ok ? 1 : 0; rg >= 0; --rg)
     for (int r : rows_c) {
         if (st != NPDE_OK) break;
         if (r > max_ny && r != rows_c[0]) continue;
         for (int w : wpb_c) {
+            if (rg && w != wpb_c[0]) continue;  // a row group's workgroup size is fixed by the width
             npde_tuning t = orig;
             t.rows_per_task = r;
             t.waves_per_block = w;
+            t.row_groups = rg;
             if ((st = npde_set_tuning(s, &t)) != NPDE_OK) break;
             double ms_sum = 0.0;
             for (int rep = 0; rep < 3 && st == NPDE_OK; ++rep) {
@@ -942,7 +976,8 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
     if (st == NPDE_OK) st = push_ctl(s);
     if (st == NPDE_OK && cache_path) {
         std::ofstream out(cache_path, std::ios::app);
-        out << key.str() << " " << best.rows_per_task << " " << best.waves_per_block << "\n";
+        out << key.str() << " " << best.rows_per_task << " " << best.waves_per_block << " " << best.row_groups
+            << "\n";
     }
     if (chosen) *chosen = s->tune;
     return st;

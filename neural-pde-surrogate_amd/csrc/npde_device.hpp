@@ -18,6 +18,7 @@ namespace npde {
 constexpr int kWave = 64;
 constexpr int kHalo = 2;
 constexpr int kStripCols = kWave - 2 * kHalo;  // 60 interior columns per wavefront
+constexpr int kMaxRowWaves = 8;                  // row-group mode: widths up to 8 * 64 = 512 columns
 constexpr int kMaxLevels = 4;                  // trajectory resolutions per run
 constexpr int kNoBad = 0x7fffffff;             // "no non-physical cell" marker
 

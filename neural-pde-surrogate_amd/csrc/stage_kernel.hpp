@@ -13,6 +13,11 @@
 //   * loads of one row are four fully coalesced 512-byte reads (one per conserved plane);
 //   * ghost cells are virtual: stencil indices outside the grid are remapped (periodic wrap or
 //     transmissive clamp) before the load;
+//   * row-group mode (ROWS): for periodic-x instances whose width is 64*W columns (W <= 8), one
+//     workgroup of W wavefronts covers a whole band of rows and every lane owns a column; the edge
+//     lanes' x neighbours come from the adjacent wavefronts of the workgroup (periodically) through
+//     a few LDS slots, read as the `old` operand of the DPP shift, so no halo column is recomputed;
+//     two workgroup barriers per row order the exchanges;
 //   * stage 2 fuses the next step's dt reduction into its epilogue (to_primitive, positivity check,
 //     max(|u|+a), max(|v|+a) of U^{n+1}) and writes one partial per task; k_finish (one wavefront per
 //     instance) reduces them and advances that instance's run state.
@@ -35,6 +40,42 @@ __device__ __forceinline__ int dpp_prev(int x) {
 __device__ __forceinline__ int dpp_next(int x) {
     return __builtin_amdgcn_mov_dpp(x, 0x130 /* wave_shl:1 */, 0xF, 0xF, true);
 }
+// the same shifts with the lane that has no source (lane 0 for from_prev, 63 for from_next) taking
+// `old` instead: the neighbour value that lives in another wavefront
+__device__ __forceinline__ int dpp_prev_or(int x, int old) {
+    return __builtin_amdgcn_update_dpp(old, x, 0x138 /* wave_shr:1 */, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int dpp_next_or(int x, int old) {
+    return __builtin_amdgcn_update_dpp(old, x, 0x130 /* wave_shl:1 */, 0xF, 0xF, false);
+}
+__device__ __forceinline__ double from_prev_or(double x, double old) {
+    return __hiloint2double(dpp_prev_or(__double2hiint(x), __double2hiint(old)),
+                            dpp_prev_or(__double2loint(x), __double2loint(old)));
+}
+__device__ __forceinline__ double from_next_or(double x, double old) {
+    return __hiloint2double(dpp_next_or(__double2hiint(x), __double2hiint(old)),
+                            dpp_next_or(__double2loint(x), __double2loint(old)));
+}
+__device__ __forceinline__ Prim from_prev_or(const Prim& w, const double* o) {
+    return {from_prev_or(w.d, o[0]), from_prev_or(w.u, o[1]), from_prev_or(w.v, o[2]), from_prev_or(w.p, o[3])};
+}
+__device__ __forceinline__ Prim from_next_or(const Prim& w, const double* o) {
+    return {from_next_or(w.d, o[0]), from_next_or(w.u, o[1]), from_next_or(w.v, o[2]), from_next_or(w.p, o[3])};
+}
+__device__ __forceinline__ Side from_prev_or(const Side& s, const double* o) {
+    return {from_prev_or(s.d, o[0]),  from_prev_or(s.mn, o[1]), from_prev_or(s.mt, o[2]),
+            from_prev_or(s.e, o[3]),  from_prev_or(s.un, o[4]), from_prev_or(s.ut, o[5]),
+            from_prev_or(s.p, o[6]),  from_prev_or(s.a, o[7]),  from_prev_or(s.rd, o[8])};
+}
+__device__ __forceinline__ Cons from_next_or(const Cons& c, const double* o) {
+    return {from_next_or(c.d, o[0]), from_next_or(c.m, o[1]), from_next_or(c.n, o[2]), from_next_or(c.e, o[3])};
+}
+__device__ __forceinline__ void put(double* o, const Prim& w) { o[0] = w.d; o[1] = w.u; o[2] = w.v; o[3] = w.p; }
+__device__ __forceinline__ void put(double* o, const Cons& c) { o[0] = c.d; o[1] = c.m; o[2] = c.n; o[3] = c.e; }
+__device__ __forceinline__ void put(double* o, const Side& s) {
+    o[0] = s.d; o[1] = s.mn; o[2] = s.mt; o[3] = s.e; o[4] = s.un; o[5] = s.ut; o[6] = s.p; o[7] = s.a; o[8] = s.rd;
+}
+
 __device__ __forceinline__ double from_prev(double x) {
     return __hiloint2double(dpp_prev(__double2hiint(x)), dpp_prev(__double2loint(x)));
 }
@@ -103,14 +144,20 @@ __device__ __forceinline__ void publish_partial(Partial* __restrict__ part, int 
 }
 
 // ---------------------------------------------------------------------------------------------
-template <int SCHEME, int LIM, int STAGE, bool CORR>
-__global__ __launch_bounds__(256) void k_stage(const InstDesc* __restrict__ desc, Ctl* __restrict__ ctl,
-                                               const Task* __restrict__ tasks, int ntasks,
-                                               Partial* __restrict__ part, int* __restrict__ ndone) {
+template <int SCHEME, int LIM, int STAGE, bool CORR, bool ROWS>
+__global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* __restrict__ desc,
+                                                                Ctl* __restrict__ ctl,
+                                                                const Task* __restrict__ tasks, int ntasks,
+                                                                Partial* __restrict__ part,
+                                                                int* __restrict__ ndone) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int tid = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
-    if (tid >= ntasks) return;
-    const Task tk = tasks[tid];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wavefront in the workgroup
+    const int nw = blockDim.x / kWave;
+    // strips: one task per wavefront; row groups: one task (band) per workgroup, one partial per wavefront
+    const int task = ROWS ? int(blockIdx.x) : __builtin_amdgcn_readfirstlane(blockIdx.x * nw + wv);
+    const int tid = ROWS ? __builtin_amdgcn_readfirstlane(blockIdx.x * nw + wv) : task;
+    if (task >= ntasks) return;
+    const Task tk = tasks[task];
     Ctl* c = ctl + tk.inst;
     if (!c->active) return;
     const double dt = c->dt;
@@ -119,9 +166,18 @@ __global__ __launch_bounds__(256) void k_stage(const InstDesc* __restrict__ desc
     const long long P = D.plane;
     const double cx = D.cx, cy = D.cy;
 
-    const int i = tk.i0 - kHalo + lane;          // this lane's column
-    const int ci = remap(i, nx, D.bc_x);          // where it is loaded from (in range for every lane)
-    const bool owner = lane >= kHalo && lane < kWave - kHalo && i < nx;
+    const int i = ROWS ? wv * kWave + lane : tk.i0 - kHalo + lane;  // this lane's column
+    const int ci = ROWS ? i : remap(i, nx, D.bc_x);  // where it is loaded from (in range for every lane)
+    const bool owner = ROWS || (lane >= kHalo && lane < kWave - kHalo && i < nx);
+
+    // row groups: edge values exchanged between the wavefronts of the workgroup.  s_w holds the edge
+    // lanes' primitives of a row (double-buffered by row parity), s_rt lane 63's right-face side,
+    // s_fl lane 0's left-face flux.
+    __shared__ double s_w[2][2][kMaxRowWaves][4];
+    __shared__ double s_rt[kMaxRowWaves][9];
+    __shared__ double s_fl[kMaxRowWaves][4];
+    const int wl = ROWS ? (wv + nw - 1) % nw : 0;  // wavefront holding the columns to the left
+    const int wr = ROWS ? (wv + 1) % nw : 0;       // ... and to the right (periodic)
 
     const double* __restrict__ V = (STAGE == 1) ? D.U : D.U1;  // stencil input
     double* O = (STAGE == 1) ? D.U1 : D.U;                     // output (stage 2: in place on U)
@@ -164,6 +220,11 @@ __global__ __launch_bounds__(256) void k_stage(const InstDesc* __restrict__ desc
         }
         top = edge_side(wa, dy, +0.5);
     }
+    if constexpr (ROWS) {  // edge primitives of row j0 for the first x slopes
+        if (lane == 0) put(s_w[j0 & 1][0][wv], wa);
+        if (lane == kWave - 1) put(s_w[j0 & 1][1][wv], wa);
+        __syncthreads();
+    }
 
     double sx = 0.0, sy = 0.0;
     int bad = kNoBad;
@@ -190,15 +251,40 @@ __global__ __launch_bounds__(256) void k_stage(const InstDesc* __restrict__ desc
         const Side topn = edge_side(wb, dyb, +0.5);
 
         // x: slope of this cell from the neighbouring lanes, left-face flux here, right-face flux from lane+1
-        const Prim dxs = slope<LIM>(from_prev(wa), wa, from_next(wa));
+        Prim dxs;
+        if constexpr (ROWS)
+            dxs = slope<LIM>(from_prev_or(wa, s_w[f & 1][1][wl]), wa, from_next_or(wa, s_w[f & 1][0][wr]));
+        else
+            dxs = slope<LIM>(from_prev(wa), wa, from_next(wa));
         const Side lf = edge_side(wa, dxs, -0.5);
         const Side rt = edge_side(wa, dxs, +0.5);
-        Cons fl = flux_x<SCHEME>(from_prev(rt), lf);
+        Side rl;  // right-face side of the column to the left
+        if constexpr (ROWS) {
+            if (lane == kWave - 1) put(s_rt[wv], rt);
+            __syncthreads();
+            rl = from_prev_or(rt, s_rt[wl]);
+        } else {
+            rl = from_prev(rt);
+        }
+        Cons fl = flux_x<SCHEME>(rl, lf);
         if constexpr (CORR) {
             const Cons k = corr_x(f);
             fl = {fl.d + k.d, fl.m + k.m, fl.n + k.n, fl.e + k.e};
         }
-        const Cons fr = from_next(fl);
+        Cons fr;  // left-face flux of the column to the right
+        if constexpr (ROWS) {
+            // with it, the edge primitives of the next row (the slots of this row's parity were
+            // read before the barrier above, so the other parity is free)
+            if (lane == 0) {
+                put(s_fl[wv], fl);
+                put(s_w[(f + 1) & 1][0][wv], wb);
+            }
+            if (lane == kWave - 1) put(s_w[(f + 1) & 1][1][wv], wb);
+            __syncthreads();
+            fr = from_next_or(fl, s_fl[wr]);
+        } else {
+            fr = from_next(fl);
+        }
 
         // L = -(F_right - F_left)/dx - (G_top - G_bottom)/dy, x first then y (solver2d.hpp:139-178)
         Cons L;
@@ -265,27 +351,34 @@ __global__ __launch_bounds__(256) void k_stage(const InstDesc* __restrict__ desc
 // launch table
 using StageFn = void (*)(const InstDesc*, Ctl*, const Task*, int, Partial*, int*);
 
-template <int S, int L>
+template <int S, int L, bool R>
 constexpr StageFn stage_fn(int stage, bool corr) {
-    return stage == 1 ? (corr ? k_stage<S, L, 1, true> : k_stage<S, L, 1, false>)
-                      : (corr ? k_stage<S, L, 2, true> : k_stage<S, L, 2, false>);
+    return stage == 1 ? (corr ? k_stage<S, L, 1, true, R> : k_stage<S, L, 1, false, R>)
+                      : (corr ? k_stage<S, L, 2, true, R> : k_stage<S, L, 2, false, R>);
 }
 
-StageFn pick_stage(int scheme, int lim, int stage, bool corr) {
+template <bool R>
+StageFn pick_stage_t(int scheme, int lim, int stage, bool corr) {
     if (scheme == kRusanov) {
         switch (lim) {
-            case kLimNone: return stage_fn<kRusanov, kLimNone>(stage, corr);
-            case kLimUnlimited: return stage_fn<kRusanov, kLimUnlimited>(stage, corr);
-            case kLimMinmod: return stage_fn<kRusanov, kLimMinmod>(stage, corr);
-            default: return stage_fn<kRusanov, kLimVanLeer>(stage, corr);
+            case kLimNone: return stage_fn<kRusanov, kLimNone, R>(stage, corr);
+            case kLimUnlimited: return stage_fn<kRusanov, kLimUnlimited, R>(stage, corr);
+            case kLimMinmod: return stage_fn<kRusanov, kLimMinmod, R>(stage, corr);
+            default: return stage_fn<kRusanov, kLimVanLeer, R>(stage, corr);
         }
     }
     switch (lim) {
-        case kLimNone: return stage_fn<kHllc, kLimNone>(stage, corr);
-        case kLimUnlimited: return stage_fn<kHllc, kLimUnlimited>(stage, corr);
-        case kLimMinmod: return stage_fn<kHllc, kLimMinmod>(stage, corr);
-        default: return stage_fn<kHllc, kLimVanLeer>(stage, corr);
+        case kLimNone: return stage_fn<kHllc, kLimNone, R>(stage, corr);
+        case kLimUnlimited: return stage_fn<kHllc, kLimUnlimited, R>(stage, corr);
+        case kLimMinmod: return stage_fn<kHllc, kLimMinmod, R>(stage, corr);
+        default: return stage_fn<kHllc, kLimVanLeer, R>(stage, corr);
     }
+}
+
+// row_waves > 0: row-group kernels (W = row_waves wavefronts per workgroup); 0: strips
+StageFn pick_stage(int scheme, int lim, int stage, bool corr, int row_waves) {
+    return row_waves > 0 ? pick_stage_t<true>(scheme, lim, stage, corr)
+                         : pick_stage_t<false>(scheme, lim, stage, corr);
 }
 
 

@@ -303,3 +303,31 @@ def test_fast_arithmetic_conserves_periodic_totals(npde_mod):
         U = b.get_state(0)
     for q in range(4):
         assert abs(U[q].sum() - U0[q].sum()) <= 1e-12 * np.abs(U0[q]).sum(), q
+
+
+@pytest.mark.parametrize("nx,nys", [(64, [64, 40]), (128, [96, 128, 33]), (512, [512])])
+def test_row_groups_match_oracle_and_strips(npde_mod, oracle, nx, nys):
+    """Row-group mode (periodic x, width 64*W: one workgroup per band, edge columns exchanged through
+    LDS) gives the oracle's bits, the same as the strip mode, for W = 1, 2 and 8 and ragged band
+    heights; the learned-increment kernels too."""
+    from npde.families import riemann
+    T = 0.01
+    cases = []
+    for k, ny in enumerate(nys):
+        U = riemann(k + 5, max(nx, ny))[0][:, :ny, :nx]
+        cases.append((oracle2d.Grid.make(nx, ny, y=(0.0, ny / nx), bc_x="periodic", bc_y=k % 2), np.ascontiguousarray(U)))
+    results = {}
+    for rg, rows in ((1, 0), (1, 7), (0, 16)):
+        with npde_mod.Batch([to_npde_grid(npde_mod, g) for g, _ in cases], "hllc", "vanleer", 0.5) as b:
+            b.set_tuning(row_groups=rg, rows_per_task=rows)
+            for k, (_, U) in enumerate(cases):
+                b.set_state(k, U)
+            b.advance_to(T)
+            results[(rg, rows)] = [(b.time(k), b.get_state(k)) for k in range(len(cases))]
+    for k, (g, U) in enumerate(cases):
+        a = U.copy()
+        ro = oracle.run(g, "hllc", "vanleer", 0.5, a, T)
+        for key, res in results.items():
+            (t, steps), got = res[k]
+            assert steps == ro["steps"] and t == ro["t"], (key, k)
+            assert same(got, a), (key, k)
