@@ -62,9 +62,10 @@ typedef struct {
     int waves_per_block;  /* wavefronts per workgroup: 1, 2, 4 (or 8 within the 256-lane bound) */
     int steps_per_graph;  /* RK2 steps captured per hipGraph launch in npde_run / npde_advance_to */
     int use_graph;        /* 0: plain stream launches */
-    int row_groups;       /* 1: when every instance is periodic in x with the same width 64*W (W <= 8), one
-                             workgroup of W wavefronts per band exchanges edge columns through LDS instead
-                             of recomputing a halo; 0: 60-column strips with a 2-column halo */
+    int row_groups;       /* when every instance is periodic in x with the same width 64*W, one workgroup of
+                             W wavefronts per band exchanges edge columns through LDS instead of recomputing
+                             a halo: 1 (default) for W <= 4, 2 for W <= 8; 0: always 60-column strips with a
+                             2-column halo */
 } npde_tuning;
 
 typedef struct npde_solver npde_solver;
@@ -87,6 +88,8 @@ npde_status npde_get_state(npde_solver* s, int instance, double* U, int layout);
 npde_status npde_set_state_device(npde_solver* s, int instance, const double* dU);
 npde_status npde_get_state_device(npde_solver* s, int instance, double* dU);
 npde_status npde_get_time(npde_solver* s, int instance, double* t, long long* steps);
+/* The same for every instance at once (arrays of npde_num_instances entries; either may be NULL). */
+npde_status npde_get_times(npde_solver* s, double* t, long long* steps);
 /* Where the library keeps an instance's state: *U = device address of its 4 planes, *plane = elements
  * between planes (nx*ny rounded up to a multiple of 64), *instance_stride = elements from this instance's
  * state to the next instance's.  For callers that read (or, between calls, write) the state in place

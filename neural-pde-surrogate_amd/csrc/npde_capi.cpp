@@ -121,12 +121,16 @@ int band_rows(const npde_solver* s) {
     return r;
 }
 
-// Row groups apply when every instance is periodic in x with the same width 64*W, W <= 8: then one
-// workgroup of W wavefronts covers a band and no halo column is recomputed.
+// Row groups apply when every instance is periodic in x with the same width 64*W: then one workgroup
+// of W wavefronts covers a band and no halo column is recomputed.  row_groups = 1 (the default) uses
+// them for W <= 4, where they measure faster than strips (64^2-256^2: +12-25%); at W = 8 a workgroup
+// of 8 wavefronts with 135-164 VGPRs is alone on its CU and waits at every barrier, and strips are
+// faster, so W <= 8 needs row_groups = 2.
 int row_group_waves_for(const npde_solver* s, const npde_tuning& tune) {
-    if (!tune.row_groups || s->n == 0) return 0;
+    if (tune.row_groups <= 0 || s->n == 0) return 0;
     const int nx = s->grids[0].nx;
-    if (nx % kWave != 0 || nx / kWave > kMaxRowWaves) return 0;
+    const int wmax = tune.row_groups >= 2 ? kMaxRowWaves : kMaxRowWaves / 2;
+    if (nx % kWave != 0 || nx / kWave > wmax) return 0;
     for (const auto& g : s->grids)
         if (g.nx != nx || g.bc_x != NPDE_BC_PERIODIC) return 0;
     return nx / kWave;
@@ -546,6 +550,17 @@ npde_status npde_get_time(npde_solver* s, int inst, double* t, long long* steps)
     return NPDE_OK;
 }
 
+npde_status npde_get_times(npde_solver* s, double* t, long long* steps) {
+    if (!s || (!t && !steps)) return NPDE_ERR_INVALID;
+    npde_status st = pull_ctl(s);
+    if (st != NPDE_OK) return st;
+    for (int k = 0; k < s->n; ++k) {
+        if (t) t[k] = s->hctl[k].t;
+        if (steps) steps[k] = s->hctl[k].steps;
+    }
+    return NPDE_OK;
+}
+
 npde_status npde_state_pointer(npde_solver* s, int inst, double** U, long long* plane, long long* instance_stride) {
     if (!s || inst < 0 || inst >= s->n || !U) return fail(s, NPDE_ERR_INVALID, "bad arguments");
     *U = s->hdesc[inst].U;
@@ -938,9 +953,9 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
     const int rows_c[] = {4, 8, 16, 32, 64, 128};
     const int wpb_c[] = {1, 2, 4};
     npde_tuning probe = orig;
-    probe.row_groups = 1;
+    probe.row_groups = 2;
     const bool rows_ok = row_group_waves_for(s, probe) > 0;  // row groups apply to this batch
-    for (int rg = rows_ok ? 1 : 0; rg >= 0; --rg)
+    for (int rg = rows_ok ? 2 : 0; rg >= 0; rg -= 2)
     for (int r : rows_c) {
         if (st != NPDE_OK) break;
         if (r > max_ny && r != rows_c[0]) continue;

@@ -50,6 +50,7 @@ _SIGNATURES = [
     ("npde_set_state_device", C.c_int, [_H, C.c_int, C.c_void_p]),
     ("npde_get_state_device", C.c_int, [_H, C.c_int, C.c_void_p]),
     ("npde_get_time", C.c_int, [_H, C.c_int, _D, C.POINTER(C.c_longlong)]),
+    ("npde_get_times", C.c_int, [_H, _D, C.POINTER(C.c_longlong)]),
     ("npde_state_pointer", C.c_int, [_H, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_longlong),
                                      C.POINTER(C.c_longlong)]),
     ("npde_compute_dt", C.c_int, [_H, _D, C.POINTER(BadCell)]),

@@ -230,31 +230,41 @@ def learned_rollout(model: FluxCorrector, indices, n=COARSE, batch=64, force_fal
                 frames = torch.empty(B, NOUT + 1, 4, n, n, dtype=torch.float32, device=dev)
                 frames[:, 0] = U.to(torch.float32)
                 keep = torch.ones(B, dtype=torch.bool, device=dev)
-                model.increments_into(U, CX, CY, keep, packed)  # warm-up (allocations, kernel loads)
-                b.advance_to(0.0)  # warm-up: captures the solver's graph outside the timing
-                net = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(net):
-                    model.increments_into(U, CX, CY, keep, packed)
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                for j in range(1, NOUT + 1):
-                    net.replay()
-                    try:
-                        b.advance_to(tf[j])
-                    except NonPhysicalStateError:
-                        # every instance that stopped short of the frame time met a non-physical state;
-                        # their remaining frames come from the fine solve
-                        for k in range(B):
-                            if b.time(k)[0] != tf[j]:
-                                keep[k] = False
-                    frames[:, j] = U.to(torch.float32)
+
+                def check_and_correct():
+                    # positivity / indicator check of the state just reached, then the increments of the
+                    # next interval (zero for instances that failed): one captured graph per interval
                     rho = U[:, 0]
                     p = (1.4 - 1.0) * (U[:, 3] - 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2) / U[:, 0])
                     ok = (torch.isfinite(U).flatten(1).all(1) & (rho.flatten(1).amin(1) > 0)
                           & (p.flatten(1).amin(1) > 0) & (rho.flatten(1).amax(1) <= BLOWUP * rho_max))
-                    keep &= ok
+                    keep.logical_and_(ok)
+                    model.increments_into(U, CX, CY, keep, packed)
+
+                check_and_correct()  # warm-up (allocations, kernel loads); U is the initial state
+                b.advance_to(0.0)  # warm-up: captures the solver's graph outside the timing
+                net = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(net):
+                    check_and_correct()
+                keep.fill_(True)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                net.replay()  # increments of the first interval
+                for j in range(1, NOUT + 1):
+                    try:
+                        b.advance_to(tf[j])
+                    except NonPhysicalStateError:
+                        # every instance that stopped short of the frame time met a non-physical state;
+                        # its remaining frames come from the fine solve, and its state is reset to a
+                        # physical one so that it does not stop the following device loops again
+                        t_now, _ = b.times()
+                        for k in np.nonzero(t_now != tf[j])[0]:
+                            keep[int(k)] = False
+                            b.set_state(int(k), U0[k])
+                    frames[:, j] = U.to(torch.float32)
                     if force_fallback:
                         keep.zero_()
+                    net.replay()  # check frame j, increments of interval j+1
                 torch.cuda.synchronize()
                 secs += time.perf_counter() - t0
                 out = frames.cpu().numpy()

@@ -218,6 +218,14 @@ class Batch:
         check(self._h, self._lib.npde_get_time(self._h, inst, C.byref(t), C.byref(steps)))
         return t.value, steps.value
 
+    def times(self):
+        """(t, steps) arrays of every instance, read in one device-to-host copy."""
+        t = np.empty(self.n)
+        steps = np.empty(self.n, dtype=np.int64)
+        check(self._h, self._lib.npde_get_times(self._h, t.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 steps.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return t, steps
+
     # -- hot path --
     def compute_dts(self) -> np.ndarray:
         dts = np.empty(self.n)

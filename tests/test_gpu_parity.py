@@ -317,7 +317,7 @@ def test_row_groups_match_oracle_and_strips(npde_mod, oracle, nx, nys):
         U = riemann(k + 5, max(nx, ny))[0][:, :ny, :nx]
         cases.append((oracle2d.Grid.make(nx, ny, y=(0.0, ny / nx), bc_x="periodic", bc_y=k % 2), np.ascontiguousarray(U)))
     results = {}
-    for rg, rows in ((1, 0), (1, 7), (0, 16)):
+    for rg, rows in ((2, 0), (2, 7), (1, 5), (0, 16)):
         with npde_mod.Batch([to_npde_grid(npde_mod, g) for g, _ in cases], "hllc", "vanleer", 0.5) as b:
             b.set_tuning(row_groups=rg, rows_per_task=rows)
             for k, (_, U) in enumerate(cases):
