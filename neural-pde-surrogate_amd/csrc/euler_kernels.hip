@@ -132,8 +132,9 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_wavespeed(const InstDe
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int nw = blockDim.x / kWave;
-    const int task = ROWS ? int(blockIdx.x) : __builtin_amdgcn_readfirstlane(blockIdx.x * nw + wv);
-    const int tid = ROWS ? __builtin_amdgcn_readfirstlane(blockIdx.x * nw + wv) : task;
+    const int blk = xcd_block(int(blockIdx.x), int(gridDim.x));
+    const int task = ROWS ? blk : __builtin_amdgcn_readfirstlane(blk * nw + wv);
+    const int tid = ROWS ? __builtin_amdgcn_readfirstlane(blk * nw + wv) : task;
     if (task >= ntasks) return;
     const Task tk = tasks[task];
     Ctl* c = ctl + tk.inst;

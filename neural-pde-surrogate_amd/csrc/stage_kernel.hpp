@@ -130,6 +130,16 @@ __device__ __forceinline__ void store4(double* B, long long off, long long P, co
 
 __device__ __forceinline__ Prim prim_of(const Cons& c) { return prim_of(c.d, c.m, c.n, c.e); }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on XCD b % 8), and each XCD has
+// its own L2.  Renumbering them so that every XCD receives one contiguous range of the task list keeps
+// neighbouring tasks, whose halo rows and columns overlap, in the same L2.  A bijection of [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    constexpr int kXcds = 8;
+    const int per = nb / kXcds, rem = nb % kXcds;
+    const int x = b % kXcds, k = b / kXcds;
+    return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+
 // This task's maxima (all lanes) -> its partial slot.
 __device__ __forceinline__ void publish_partial(Partial* __restrict__ part, int tid, int lane, double sx,
                                                 double sy, int bad) {
@@ -154,8 +164,9 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* 
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wavefront in the workgroup
     const int nw = blockDim.x / kWave;
     // strips: one task per wavefront; row groups: one task (band) per workgroup, one partial per wavefront
-    const int task = ROWS ? int(blockIdx.x) : __builtin_amdgcn_readfirstlane(blockIdx.x * nw + wv);
-    const int tid = ROWS ? __builtin_amdgcn_readfirstlane(blockIdx.x * nw + wv) : task;
+    const int blk = xcd_block(int(blockIdx.x), int(gridDim.x));
+    const int task = ROWS ? blk : __builtin_amdgcn_readfirstlane(blk * nw + wv);
+    const int tid = ROWS ? __builtin_amdgcn_readfirstlane(blk * nw + wv) : task;
     if (task >= ntasks) return;
     const Task tk = tasks[task];
     Ctl* c = ctl + tk.inst;

@@ -24,6 +24,10 @@ void write_trajectory(const std::string&, const npde_grid_desc&, const std::vect
                       const RunInfo&, double, double) {
     no_hdf5();
 }
+void write_packed(const std::string&, int, int, int, const float*, const double*, const std::vector<std::string>&,
+                  const int*, const std::string&, double) {
+    no_hdf5();
+}
 
 #else
 
@@ -245,6 +249,50 @@ void write_trajectory(const std::string& path, const npde_grid_desc& g, const st
     attr(f.id, "test", info.test);
     attr(f.id, "scheme", info.scheme);
     attr(f.id, "limiter", info.limiter);
+}
+
+void write_packed(const std::string& path, int N, int F, int n, const float* U, const double* t,
+                  const std::vector<std::string>& family, const int* index, const std::string& split, double dt) {
+    if (N <= 0 || F <= 0 || n <= 0 || int(family.size()) != N) throw std::runtime_error("bad packed dataset shape");
+    H f(H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose);
+    if (f.id < 0) throw std::runtime_error("could not create " + path + " (does the folder exist?)");
+    {
+        hsize_t dims[5] = {hsize_t(N), hsize_t(F), 4, hsize_t(n), hsize_t(n)};
+        hsize_t chunk[5] = {1, 1, 4, hsize_t(n), hsize_t(n)};
+        H sp(H5Screate_simple(5, dims, nullptr), H5Sclose);
+        H pl(H5Pcreate(H5P_DATASET_CREATE), H5Pclose);
+        H5Pset_chunk(pl.id, 5, chunk);
+        H5Pset_shuffle(pl.id);
+        H5Pset_deflate(pl.id, 1);
+        H d(H5Dcreate2(f.id, "U", H5T_NATIVE_FLOAT, sp.id, H5P_DEFAULT, pl.id, H5P_DEFAULT), H5Dclose);
+        if (d.id < 0 || H5Dwrite(d.id, H5T_NATIVE_FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, U) < 0)
+            throw std::runtime_error("could not write dataset U");
+    }
+    dataset(f.id, "t", t, {hsize_t(N), hsize_t(F)});
+    {
+        std::vector<const char*> ptrs(N);
+        for (int k = 0; k < N; ++k) ptrs[k] = family[k].c_str();
+        H ty(H5Tcopy(H5T_C_S1), H5Tclose);
+        H5Tset_size(ty.id, H5T_VARIABLE);
+        H5Tset_cset(ty.id, H5T_CSET_UTF8);
+        hsize_t dims[1] = {hsize_t(N)};
+        H sp(H5Screate_simple(1, dims, nullptr), H5Sclose);
+        H d(H5Dcreate2(f.id, "family", ty.id, sp.id, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose);
+        if (d.id < 0 || H5Dwrite(d.id, ty.id, H5S_ALL, H5S_ALL, H5P_DEFAULT, ptrs.data()) < 0)
+            throw std::runtime_error("could not write dataset family");
+    }
+    {
+        hsize_t dims[1] = {hsize_t(N)};
+        H sp(H5Screate_simple(1, dims, nullptr), H5Sclose);
+        H d(H5Dcreate2(f.id, "index", H5T_STD_I32LE, sp.id, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose);
+        if (d.id < 0 || H5Dwrite(d.id, H5T_NATIVE_INT, H5S_ALL, H5S_ALL, H5P_DEFAULT, index) < 0)
+            throw std::runtime_error("could not write dataset index");
+    }
+    attr(f.id, "variables", std::string("rho, rho_u, rho_v, E"));
+    attr(f.id, "split", split);
+    attr(f.id, "n", n);
+    attr(f.id, "frames", F);
+    attr(f.id, "dt", dt);
 }
 
 #endif

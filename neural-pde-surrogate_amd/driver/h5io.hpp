@@ -39,4 +39,10 @@ struct Level {
 void write_trajectory(const std::string& path, const npde_grid_desc& g, const std::vector<double>& times,
                       const std::vector<Level>& levels, const RunInfo& info, double t_end, double t_start);
 
+// Packed dataset file of one (split, resolution), the layout of the reference's data/pack.py:55-80:
+//   U (N, F, 4, n, n) float32 chunked (1, 1, 4, n, n), shuffle + deflate 1; t (N, F) binary64;
+//   family (N) variable-length strings; index (N) int32; attributes variables, split, n, frames, dt.
+void write_packed(const std::string& path, int N, int F, int n, const float* U, const double* t,
+                  const std::vector<std::string>& family, const int* index, const std::string& split, double dt);
+
 }  // namespace npde_io

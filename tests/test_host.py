@@ -26,6 +26,23 @@ def test_library_exports_every_declared_symbol():
     assert sorted(L.EXPORTED) == declared
 
 
+def test_io_library_exports_every_declared_symbol():
+    """libnpde_io.so (dataset packer) exports what include/npde_io.h declares."""
+    import ctypes as C
+    import re
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1]
+    so = root / "neural-pde-surrogate_amd" / "lib" / "libnpde_io.so"
+    if not so.exists():
+        pytest.skip("libnpde_io.so not built")
+    names = re.findall(r"^\s*(?:int|const char\*)\s+(npde_io_\w+)\(", (root / "include" / "npde_io.h").read_text(),
+                       re.M)
+    lib = C.CDLL(str(so))
+    assert sorted(names) == ["npde_io_available", "npde_io_last_error", "npde_io_write_packed"]
+    for n in names:
+        assert hasattr(lib, n), n
+
+
 def test_builtin_problem_on_host_matches_oracle(oracle):
     """npde_builtin_problem is host code (no GPU): same ICs as the reference's setup()."""
     import npde
