@@ -1,0 +1,315 @@
+"""Full-state surrogates: ResNet, U-Net and FNO mapping one normalised state (B, 4, n, n) to the next.
+
+The architectures of the reference's surrogate/models.py (Equinox), rebuilt on torch for the GPU:
+
+  ResNet  (models.py:48-79)    1x1 lift -> `depth` pre-activation residual blocks of two k x k periodic
+                               convolutions -> 1x1 projection; predicts the one-frame change,
+                               next = x + delta_scale * net(x); zero-initialised projection.
+  UNet    (models.py:80-152)   ConvBlocks down `levels` halvings (2x2 average pooling) and back up
+                               (nearest-neighbour upsampling, skip concatenation); channels double per level.
+  FNO     (models.py:155-248)  spectral convolutions keeping |kx| < modes, 0 <= ky < modes (two complex weight
+                               blocks for kx >= 0 and kx < 0) plus a 1x1 path, GELU; lift / project 1x1.
+  Positive (models.py:259-300) wrapper working in (log rho, u, v, log p): density and pressure of the output
+                               are positive by construction.
+
+All convolutions use periodic padding, so the ResNet and FNO are exactly translation-equivariant on the
+torus (the U-Net for shifts that are multiples of 2**levels).  GELU is the tanh approximation (JAX's
+default, which the reference uses).  Parameter shapes follow the reference, so parameter counts agree:
+ResNet(64, 8) 591,428; UNet(16, 3) 516,516; FNO(32, 4, 16 modes) 4,203,428.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+import torch.nn.functional as F
+
+GAMMA = 1.4
+TINY = 1e-6
+
+
+def gelu(x):
+    return F.gelu(x, approximate="tanh")
+
+
+class PeriodicConv(torch.nn.Module):
+    """k x k convolution with wrap-around padding (models.py:16-29).  Weights are initialised like
+    Equinox's Conv2d: uniform in +-1/sqrt(fan_in)."""
+
+    def __init__(self, c_in: int, c_out: int, k: int, generator: torch.Generator | None = None):
+        super().__init__()
+        bound = 1.0 / math.sqrt(c_in * k * k)
+        self.weight = torch.nn.Parameter((torch.rand(c_out, c_in, k, k, generator=generator) * 2 - 1) * bound)
+        self.bias = torch.nn.Parameter((torch.rand(c_out, generator=generator) * 2 - 1) * bound)
+        self.pad = k // 2
+
+    def zero_(self):
+        with torch.no_grad():
+            self.weight.zero_()
+            self.bias.zero_()
+        return self
+
+    def forward(self, x):
+        if self.pad:
+            p = self.pad
+            x = F.pad(x, (p, p, p, p), mode="circular")
+        return F.conv2d(x, self.weight, self.bias)
+
+
+def _scale(delta_scale, channels):
+    return torch.tensor((1.0,) * channels if delta_scale is None else tuple(float(d) for d in delta_scale))
+
+
+class ResBlock(torch.nn.Module):
+    """x + conv(gelu(conv(gelu(x))))."""
+
+    def __init__(self, width, k, generator=None):
+        super().__init__()
+        self.conv1 = PeriodicConv(width, width, k, generator)
+        self.conv2 = PeriodicConv(width, width, k, generator)
+
+    def forward(self, x):
+        return x + self.conv2(gelu(self.conv1(gelu(x))))
+
+
+class ResNet(torch.nn.Module):
+    def __init__(self, width=64, depth=8, kernel_size=3, delta_scale=None, channels=4, seed=0):
+        super().__init__()
+        g = torch.Generator().manual_seed(seed)
+        self.lift = PeriodicConv(channels, width, 1, g)
+        self.blocks = torch.nn.ModuleList([ResBlock(width, kernel_size, g) for _ in range(depth)])
+        self.project = PeriodicConv(width, channels, 1, g).zero_()
+        self.register_buffer("delta_scale", _scale(delta_scale, channels))
+
+    def forward(self, x):
+        h = self.lift(x)
+        for b in self.blocks:
+            h = b(h)
+        return x + self.delta_scale.view(1, -1, 1, 1).to(x.dtype) * self.project(gelu(h))
+
+
+class ConvBlock(torch.nn.Module):
+    """skip(x) + conv(gelu(conv(gelu(x)))), with a 1x1 skip when the channel count changes."""
+
+    def __init__(self, c_in, c_out, generator=None):
+        super().__init__()
+        self.conv1 = PeriodicConv(c_in, c_out, 3, generator)
+        self.conv2 = PeriodicConv(c_out, c_out, 3, generator)
+        self.skip = None if c_in == c_out else PeriodicConv(c_in, c_out, 1, generator)
+
+    def forward(self, x):
+        h = self.conv2(gelu(self.conv1(gelu(x))))
+        return (x if self.skip is None else self.skip(x)) + h
+
+
+def downsample(x):
+    """2x2 average pooling (the dataset's own coarsening)."""
+    B, c, n, m = x.shape
+    return x.reshape(B, c, n // 2, 2, m // 2, 2).mean(dim=(3, 5))
+
+
+def upsample(x):
+    """Nearest-neighbour upsampling by 2."""
+    return x.repeat_interleave(2, dim=-2).repeat_interleave(2, dim=-1)
+
+
+class UNet(torch.nn.Module):
+    def __init__(self, width=16, levels=3, delta_scale=None, channels=4, seed=0):
+        super().__init__()
+        g = torch.Generator().manual_seed(seed)
+        ch = [width * 2 ** l for l in range(levels + 1)]
+        self.lift = PeriodicConv(channels, ch[0], 1, g)
+        self.down = torch.nn.ModuleList([ConvBlock(ch[max(l - 1, 0)], ch[l], g) for l in range(levels)])
+        self.bottleneck = ConvBlock(ch[levels - 1], ch[levels], g)
+        self.up = torch.nn.ModuleList([ConvBlock(ch[l + 1] + ch[l], ch[l], g) for l in range(levels)])
+        self.project = PeriodicConv(ch[0], channels, 1, g).zero_()
+        self.register_buffer("delta_scale", _scale(delta_scale, channels))
+
+    def forward(self, x):
+        h = self.lift(x)
+        skips = []
+        for block in self.down:
+            h = block(h)
+            skips.append(h)
+            h = downsample(h)
+        h = self.bottleneck(h)
+        for block, skip in zip(reversed(self.up), reversed(skips)):
+            h = block(torch.cat([upsample(h), skip], dim=1))
+        return x + self.delta_scale.view(1, -1, 1, 1).to(x.dtype) * self.project(gelu(h))
+
+
+class SpectralConv2d(torch.nn.Module):
+    """Convolution as a multiplication of the lowest Fourier modes (models.py:155-190): rfft2, keep
+    kx in [0, modes) and [-modes, 0), ky in [0, modes), mix channels with complex weights, irfft2."""
+
+    def __init__(self, c_in, c_out, modes, generator=None):
+        super().__init__()
+        scale = 1.0 / (c_in * c_out)
+        shape = (2, c_in, c_out, modes, modes)
+        self.w_re = torch.nn.Parameter(scale * torch.rand(shape, generator=generator))
+        self.w_im = torch.nn.Parameter(scale * torch.rand(shape, generator=generator))
+        self.modes = modes
+
+    def forward(self, x):
+        B, c_in, n, _ = x.shape
+        m = self.modes
+        X = torch.fft.rfft2(x)
+        W = torch.complex(self.w_re, self.w_im).to(X.dtype)
+        Y = torch.zeros(B, W.shape[2], n, n // 2 + 1, dtype=X.dtype, device=x.device)
+        Y[:, :, :m, :m] = torch.einsum("bixy,ioxy->boxy", X[:, :, :m, :m], W[0])
+        Y[:, :, -m:, :m] = torch.einsum("bixy,ioxy->boxy", X[:, :, -m:, :m], W[1])
+        return torch.fft.irfft2(Y, s=(n, n))
+
+
+class FNOBlock(torch.nn.Module):
+    def __init__(self, width, modes, generator=None):
+        super().__init__()
+        self.spectral = SpectralConv2d(width, width, modes, generator)
+        self.pointwise = PeriodicConv(width, width, 1, generator)
+
+    def forward(self, h):
+        return gelu(self.spectral(h) + self.pointwise(h))
+
+
+class FNO(torch.nn.Module):
+    def __init__(self, width=32, depth=4, modes=12, delta_scale=None, channels=4, hidden=128, seed=0):
+        super().__init__()
+        g = torch.Generator().manual_seed(seed)
+        self.lift = PeriodicConv(channels, width, 1, g)
+        self.blocks = torch.nn.ModuleList([FNOBlock(width, modes, g) for _ in range(depth)])
+        self.project1 = PeriodicConv(width, hidden, 1, g)
+        self.project2 = PeriodicConv(hidden, channels, 1, g).zero_()
+        self.register_buffer("delta_scale", _scale(delta_scale, channels))
+
+    def forward(self, x):
+        h = self.lift(x)
+        for b in self.blocks:
+            h = b(h)
+        return x + self.delta_scale.view(1, -1, 1, 1).to(x.dtype) * self.project2(gelu(self.project1(h)))
+
+
+def conserved_to_logprim(U):
+    """(rho, rho u, rho v, E) -> (log rho, u, v, log p); U (B, 4, n, n) physical."""
+    rho = torch.clamp(U[:, 0], min=TINY)
+    u, v = U[:, 1] / rho, U[:, 2] / rho
+    p = (GAMMA - 1) * (U[:, 3] - 0.5 * rho * (u ** 2 + v ** 2))
+    return torch.stack([torch.log(rho), u, v, torch.log(torch.clamp(p, min=TINY))], dim=1)
+
+
+def logprim_to_conserved(W):
+    """(log rho, u, v, log p) -> (rho, rho u, rho v, E), positive density, pressure and energy."""
+    rho, u, v, p = torch.exp(W[:, 0]), W[:, 1], W[:, 2], torch.exp(W[:, 3])
+    return torch.stack([rho, rho * u, rho * v, p / (GAMMA - 1) + 0.5 * rho * (u ** 2 + v ** 2)], dim=1)
+
+
+class Positive(torch.nn.Module):
+    """Wraps a core model so that it steps (log rho, u, v, log p) standardised with train statistics;
+    input and output stay normalised conserved variables (models.py:259-300)."""
+
+    def __init__(self, core, u_mean, u_std, w_mean, w_std):
+        super().__init__()
+        self.core = core
+        for name, v in (("u_mean", u_mean), ("u_std", u_std), ("w_mean", w_mean), ("w_std", w_std)):
+            self.register_buffer(name, torch.as_tensor(v, dtype=torch.float32).view(1, 4, 1, 1))
+
+    def forward(self, x):
+        U = x * self.u_std + self.u_mean
+        z = (conserved_to_logprim(U) - self.w_mean) / self.w_std
+        W = self.core(z) * self.w_std + self.w_mean
+        return (logprim_to_conserved(W) - self.u_mean) / self.u_std
+
+
+def count_params(model: torch.nn.Module) -> int:
+    return sum(p.numel() for p in model.parameters())
+
+
+MODELS = {"resnet": ResNet, "unet": UNet, "fno": FNO}
+
+
+# ------------------------------------------------------------------------------------------- training
+class Normaliser:
+    """Per-variable (U - mean) / std with train-split statistics (stats_<n>.json of the dataset pipeline)."""
+
+    def __init__(self, mean, std, delta_std):
+        self.mean = torch.as_tensor(mean, dtype=torch.float32).view(1, 4, 1, 1)
+        self.std = torch.as_tensor(std, dtype=torch.float32).view(1, 4, 1, 1)
+        self.delta_std = torch.as_tensor(delta_std, dtype=torch.float32)
+
+    @classmethod
+    def from_stats(cls, stats: dict):
+        return cls(stats["mean"], stats["std"], stats["delta_std"])
+
+    def encode(self, U):
+        return (U - self.mean.to(U.device)) / self.std.to(U.device)
+
+    def decode(self, Z):
+        return Z * self.std.to(Z.device) + self.mean.to(Z.device)
+
+
+def make_pairs(n_traj, n_frames, horizon):
+    """Every (trajectory, start frame) whose `horizon` targets stay inside the trajectory."""
+    return torch.tensor([(i, t) for i in range(n_traj) for t in range(n_frames - horizon)], dtype=torch.long)
+
+
+def loss_fn(model, x, y, mode="onestep"):
+    """x (B, 4, n, n); y (B, k, 4, n, n).  onestep: MSE of one step to y[:, 0]; multistep: mean MSE over k
+    self-fed steps; pushforward: k-1 steps without gradient, then one trained step to y[:, -1]
+    (train.py:89-111)."""
+    if mode == "onestep":
+        return torch.mean((model(x) - y[:, 0]) ** 2)
+    if mode == "multistep":
+        losses = []
+        for j in range(y.shape[1]):
+            x = model(x)
+            losses.append(torch.mean((x - y[:, j]) ** 2))
+        return torch.stack(losses).mean()
+    with torch.no_grad():
+        for _ in range(y.shape[1] - 1):
+            x = model(x)
+    return torch.mean((model(x) - y[:, -1]) ** 2)
+
+
+def train(model, frames, norm: Normaliser, steps=200, batch=16, lr=1e-3, warmup=20, horizon=1, mode="onestep",
+          seed=0, log=print):
+    """frames: (N, F, 4, n, n) float32 tensor of train trajectories on the model's device.  AdamW with
+    warm-up + cosine decay and global-norm clipping 1.0 (the reference's optimiser, train.py:209-211).
+    Returns the list of losses."""
+    opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=1e-4)
+    sched = torch.optim.lr_scheduler.LambdaLR(
+        opt, lambda i: min(1.0, (i + 1) / warmup) * 0.5 * (1 + math.cos(math.pi * min(1.0, i / steps))))
+    pairs = make_pairs(frames.shape[0], frames.shape[1], horizon)
+    g = torch.Generator().manual_seed(seed)
+    hist = []
+    for it in range(steps):
+        sel = pairs[torch.randint(0, len(pairs), (batch,), generator=g)]
+        x = norm.encode(frames[sel[:, 0], sel[:, 1]])
+        y = torch.stack([norm.encode(frames[sel[:, 0], sel[:, 1] + j]) for j in range(1, horizon + 1)], dim=1)
+        loss = loss_fn(model, x, y, mode)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        torch.nn.utils.clip_grad_norm_(model.parameters(), 1.0)
+        opt.step()
+        sched.step()
+        hist.append(float(loss.detach()))
+        if it % 50 == 0 or it == steps - 1:
+            log(f"  step {it:5d}  loss {hist[-1]:.3e}")
+    return hist
+
+
+@torch.no_grad()
+def rollout(model, x0, steps):
+    """Self-fed rollout: (B, 4, n, n) normalised initial states -> (B, steps + 1, 4, n, n)."""
+    out = [x0]
+    x = x0
+    for _ in range(steps):
+        x = model(x)
+        out.append(x)
+    return torch.stack(out, dim=1)
+
+
+def rollout_metrics(pred, true, std):
+    """Per frame and variable RMSE over std, and density L1 (rollout.py:54-62).  pred, true physical
+    (T, 4, n, n); std (4,)."""
+    rmse = torch.sqrt(torch.mean((pred - true) ** 2, dim=(-2, -1))) / torch.as_tensor(std, device=pred.device)
+    l1_rho = torch.mean(torch.abs(pred[:, 0] - true[:, 0]), dim=(-2, -1))
+    return rmse, l1_rho
