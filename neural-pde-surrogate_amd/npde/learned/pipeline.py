@@ -340,6 +340,8 @@ def main(argv=None):
     ap.add_argument("--unroll", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--init", default=None, help="start from these weights (.npz) at the full unroll")
+    ap.add_argument("--out-scale", type=float, default=None, help="bound of the increments (fraction of the face "
+                    "flux scale) for a new corrector")
     args = ap.parse_args(argv)
     out = Path(args.out)
     out.mkdir(parents=True, exist_ok=True)
@@ -360,7 +362,7 @@ def main(argv=None):
     log(f"training: {args.iters} iterations, batch {args.batch}, unroll {args.unroll}")
     t0 = time.perf_counter()
     cur = lambda it: max(2, min(args.unroll, 2 ** (1 + (3 * it) // max(1, args.iters))))  # 2, 4, 8
-    init = None
+    init = None if args.out_scale is None else FluxCorrector(out_scale=args.out_scale)
     if args.init:  # continue a previous run: full unroll from the start
         from .model import load_trained
         init, cur = load_trained(args.init), None
