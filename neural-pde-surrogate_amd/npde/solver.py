@@ -194,7 +194,8 @@ class Batch:
 
     def state_view(self):
         """Zero-copy torch view (n, 4, ny, nx) of every instance's state in device memory (all instances
-        must have the same size).  Read it after a call returns; write it only between calls."""
+        must have the same size).  Read it after a call returns; write it only between calls.  The view
+        aliases the library's buffer: it is valid until the batch is closed."""
         import torch
         g0 = self.grids[0]
         if any((g.nx, g.ny) != (g0.nx, g0.ny) for g in self.grids):
@@ -211,7 +212,9 @@ class Batch:
                 "shape": (self.n, 4, g0.ny, g0.nx), "typestr": "<f8", "data": (ptr.value, False), "version": 3,
                 "strides": (stride.value * 8, plane.value * 8, g0.nx * 8, 8)}
 
-        return torch.as_tensor(_View(), device="cuda")
+        view = torch.as_tensor(_View(), device="cuda")
+        view._npde_owner = self  # keeps the batch (and its device buffer) alive while the view is
+        return view
 
     def time(self, inst: int = 0):
         t, steps = C.c_double(), C.c_longlong()
