@@ -169,17 +169,20 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ x,
     }
 }
 
-// dW[k] = sum over bands (in band order) of part[band][k]; db likewise.
+// dW[k] = sum over bands (in band order) of part[band][k]; db likewise.  The band loop is unrolled so
+// that many loads are in flight while the additions keep their fixed order (deterministic gradients).
 __global__ void k_wgrad_reduce(const float* __restrict__ part, const float* __restrict__ bpart, int nbands,
                                int nw, int nb, float* __restrict__ dw, float* __restrict__ db) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < nw) {
         float s = 0.f;
+#pragma unroll 16
         for (int band = 0; band < nbands; ++band) s += part[size_t(band) * nw + k];
         dw[k] = s;
     } else if (k < nw + nb) {
         const int c = k - nw;
         float s = 0.f;
+#pragma unroll 16
         for (int band = 0; band < nbands; ++band) s += bpart[size_t(band) * nb + c];
         db[c] = s;
     }

@@ -156,3 +156,38 @@ class FluxCNN(torch.autograd.Function):
         for k in range(L):
             out += [grads_w[k], grads_b[k]]
         return tuple(out)
+
+
+def main(argv=None):
+    """Time the corrector network's kernels at the training shape (forward + backward, batch 32 x 64^2)
+    and the rollout shape (forward, batch 64 x 64^2):  python -m npde.learned.conv [--iters 12]"""
+    import argparse
+    import time
+    from .model import FluxCorrector
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--iters", type=int, default=12)
+    args = ap.parse_args(argv)
+    dev = torch.device("cuda")
+    m = FluxCorrector().to(dev)
+    with torch.no_grad():
+        m.weights[-1].normal_(0.0, 0.1)
+    g = torch.Generator(device=dev).manual_seed(0)
+    for B, training in ((32, True), (64, False)):
+        x = torch.randn(B, 64, 64, 16, device=dev, generator=g)
+        x[..., 8:] = 0
+        packed = None if training else pack_stack(m.params())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.iters):
+            if training:
+                FluxCNN.apply(x, *m.params()).sum().backward()
+            else:
+                with torch.no_grad():
+                    flux_cnn_infer(x, packed)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.iters
+        print(f"batch {B} x 64^2 {'forward + backward' if training else 'forward'}: {ms:.3f} ms per call", flush=True)
+
+
+if __name__ == "__main__":
+    main()
