@@ -9,7 +9,8 @@
 // Work decomposition: a "task" is one wavefront's share of one RK stage -- a strip of 60 interior
 // columns (64 lanes minus a 2-column halo on each side) and a band of rows, marched upward row by
 // row.  Tasks of all instances form one list; a stage launch covers the whole list, and waves of
-// instances that are not stepping leave at once.
+// instances that are not stepping leave at once.  In row-group mode (periodic x, width 64*W) a task
+// is a band of all the columns, run by one workgroup of W wavefronts with one partial per wavefront.
 #pragma once
 #include <cstdint>
 

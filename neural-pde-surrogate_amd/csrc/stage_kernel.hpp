@@ -17,7 +17,11 @@
 //     workgroup of W wavefronts covers a whole band of rows and every lane owns a column; the edge
 //     lanes' x neighbours come from the adjacent wavefronts of the workgroup (periodically) through
 //     a few LDS slots, read as the `old` operand of the DPP shift, so no halo column is recomputed;
-//     two workgroup barriers per row order the exchanges;
+//     two workgroup barriers per row order the exchanges (the host uses it up to W = 4 by default:
+//     an 8-wavefront workgroup at these register counts is alone on its CU and waits at each barrier);
+//   * workgroups are renumbered so that each XCD runs a contiguous range of the task list, which
+//     keeps neighbouring tasks (overlapping halos) in one L2;
+//   * the row loop is unrolled so that the carried rows are renamed instead of copied;
 //   * stage 2 fuses the next step's dt reduction into its epilogue (to_primitive, positivity check,
 //     max(|u|+a), max(|v|+a) of U^{n+1}) and writes one partial per task; k_finish (one wavefront per
 //     instance) reduces them and advances that instance's run state.
