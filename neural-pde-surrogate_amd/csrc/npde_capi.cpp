@@ -966,14 +966,15 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
             t.waves_per_block = w;
             t.row_groups = rg;
             if ((st = npde_set_tuning(s, &t)) != NPDE_OK) break;
-            double ms_sum = 0.0;
-            for (int rep = 0; rep < 3 && st == NPDE_OK; ++rep) {
+            // the fastest of 5 timed steps (after one warm-up): robust to a stray slow step
+            double ms_min = 1e30;
+            for (int rep = 0; rep < 6 && st == NPDE_OK; ++rep) {
                 copy_state(false);
                 st = npde_step(s, dts.data());
-                if (rep > 0) ms_sum += s->last_ms;  // first is warm-up
+                if (rep > 0) ms_min = std::min(ms_min, s->last_ms);
             }
-            if (st == NPDE_OK && ms_sum < best_ms) {
-                best_ms = ms_sum;
+            if (st == NPDE_OK && ms_min < best_ms) {
+                best_ms = ms_min;
                 best = t;
             }
         }

@@ -331,3 +331,27 @@ def test_row_groups_match_oracle_and_strips(npde_mod, oracle, nx, nys):
             (t, steps), got = res[k]
             assert steps == ro["steps"] and t == ro["t"], (key, k)
             assert same(got, a), (key, k)
+
+
+def test_trajectory_invariants_of_check_trajectory(npde_mod):
+    """tests/check_trajectory.py:51-74 on a device-loop trajectory (periodic 128^2 random Riemann,
+    levels 64/32/16): equally spaced frame times, every coarse level the block average of the finest to
+    1e-6, totals equal across levels and constant in time, positive density and pressure."""
+    from npde.families import riemann
+    with npde_mod.Batch([npde_mod.make_grid(128, 128, bc_x=1, bc_y=1)], "hllc", "vanleer", 0.5) as b:
+        b.set_state(0, riemann(11, 128)[0])
+        b.run_trajectory(0.2, 10, sizes=(64, 32, 16))
+        t = b.frame_times(0)
+        L = {m: b.trajectory(0, m).astype(np.float64) for m in (64, 32, 16)}
+    assert np.allclose(np.diff(t), 0.02, rtol=0, atol=1e-12)
+    fine = L[64]
+    for m in (32, 16):
+        f = 64 // m
+        ba = fine.reshape(11, 4, m, f, m, f).mean(axis=(3, 5))
+        assert np.abs(ba - L[m]).max() <= 1e-6 * np.abs(fine).max(), m
+        assert np.allclose(L[m].mean(axis=(2, 3)), fine.mean(axis=(2, 3)), rtol=1e-6, atol=1e-6)
+    tot = fine.mean(axis=(2, 3))
+    assert np.abs(tot - tot[0]).max() <= 1e-6 * np.abs(tot[0]).max()
+    rho = fine[:, 0]
+    p = 0.4 * (fine[:, 3] - 0.5 * (fine[:, 1] ** 2 + fine[:, 2] ** 2) / rho)
+    assert rho.min() > 0 and p.min() > 0
