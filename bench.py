@@ -60,15 +60,17 @@ def cpu_baseline(U0, seconds):
                       f"from t = 0 in {elapsed:.2f} s on {threads} OpenMP threads"}
 
 
-def traffic():
-    """HBM bytes per stage-2 launch from the committed rocprofv3 counter summary (FETCH_SIZE x 2 +
-    WRITE_SIZE x 1, the factors a plain device copy of known size gives), or None without it."""
+def traffic(layout):
+    """HBM bytes per stage-2 launch in this task layout from the committed rocprofv3 counter summary
+    (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a plain device copy of known size gives), or None
+    without a measurement of that layout."""
     f = ROOT / "profiles" / "stage_kernels_64x512_counters.md"
     if not f.exists():
         return None
+    key = f"stage2_hbm_bytes_per_launch ({layout}) = "
     for line in f.read_text().splitlines():
-        if line.startswith("stage2_hbm_bytes_per_launch = "):
-            return float(line.split("=")[1])
+        if line.startswith(key):
+            return float(line[len(key):])
     return None
 
 
@@ -126,6 +128,8 @@ def main():
     ap.add_argument("--wpb", type=int, default=None, help="fix the waves per block (skips autotuning)")
     ap.add_argument("--row-groups", type=int, default=None, choices=(0, 1, 2),
                     help="fix the row-group mode (skips autotuning)")
+    ap.add_argument("--seams", type=int, default=None, choices=(0, 1),
+                    help="fix the seam mode: 64-column strips that solve their seams first (skips autotuning)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 1 GiB device copy (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -154,7 +158,7 @@ def main():
     for k, U in enumerate(ics):
         b.set_state(k, U)
     fixed = {k: v for k, v in (("rows_per_task", args.rows), ("waves_per_block", args.wpb),
-                                ("row_groups", args.row_groups)) if v is not None}
+                                ("row_groups", args.row_groups), ("seams", args.seams)) if v is not None}
     if fixed:
         b.set_tuning(**fixed)
         tuning = b.tuning
@@ -162,7 +166,8 @@ def main():
         tuning = b.tuning if args.no_tune else b.autotune(None)
     for k, U in enumerate(ics):  # autotuning restores the state; set it again anyway
         b.set_state(k, U)
-    log(f"tuning: {tuning}")
+    layout = b.task_layout
+    log(f"tuning: {tuning}, task layout {layout}")
 
     b.advance_steps(args.warmup, 0.5)
     s0 = [b.time(k)[1] for k in range(N_INST)]
@@ -193,10 +198,10 @@ def main():
         "config": {"workload": "random-Riemann batch: 64 instances x 512^2, periodic, HLLC + van Leer MUSCL, "
                                "SSP-RK2, CFL 0.5, per-instance dt, device-resident loop (hipGraph)",
                    "instances": N_INST, "n": N, "tuning": tuning},
-        "roofline": {"bound": "hbm", "kernel": "npde::exact::k_stage<HLLC, vanLeer, 2, false>", "achieved": ach2,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach2 / HBM_PEAK_GBS, "traffic": traffic(),
-                     "bytes_per_cell": BYTES_STAGE2, "launch_ms": ms2, "stage1_launch_ms": ms1,
-                     "stage1_achieved": ach1},
+        "roofline": {"bound": "hbm", "kernel": f"npde::exact::k_stage<HLLC, vanLeer, 2, false, {layout}>",
+                     "achieved": ach2, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach2 / HBM_PEAK_GBS,
+                     "traffic": traffic(layout), "bytes_per_cell": BYTES_STAGE2, "launch_ms": ms2,
+                     "stage1_launch_ms": ms1, "stage1_achieved": ach1},
         "cpu_baseline": cpu,
         "fast_arith_variant": fast,
     }

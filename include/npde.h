@@ -41,6 +41,9 @@ typedef enum { NPDE_LAYOUT_PLANES = 0, NPDE_LAYOUT_GHOSTED = 1 } npde_layout;
  * solver.  FAST: reciprocal-based divisions, a shortened square root and fused multiply-adds; each
  * step differs from EXACT by rounding-level amounts.  No counterpart in the reference. */
 typedef enum { NPDE_ARITH_EXACT = 0, NPDE_ARITH_FAST = 1 } npde_arith;
+/* Task layout of the stage kernels (from npde_tuning and the instance widths; every layout gives the same
+ * bits): 60-column strips with a halo, row groups, or 64-column strips that solve their seams first. */
+typedef enum { NPDE_TASKS_STRIPS = 0, NPDE_TASKS_ROW_GROUPS = 1, NPDE_TASKS_SEAMS = 2 } npde_task_layout;
 
 /* Grid of one instance: replaces make_grid(nx, ny, x_min, x_max, y_min, y_max, bc_x, bc_y)
  * (grid2d.hpp:30-39). */
@@ -66,6 +69,10 @@ typedef struct {
                              W wavefronts per band exchanges edge columns through LDS instead of recomputing
                              a halo: 1 (default) for W <= 4, 2 for W <= 8; 0: always 60-column strips with a
                              2-column halo */
+    int seams;            /* when every instance's width is a multiple of 64, row groups do not apply and bands
+                             are at most 64 rows, each wavefront owns a 64-column strip and first solves the
+                             two strip boundaries of its band (from edge columns the previous stage kernel
+                             wrote): 1 (default); 0: 60-column strips with a 2-column halo */
 } npde_tuning;
 
 typedef struct npde_solver npde_solver;
@@ -163,6 +170,8 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
 /* Select the stage kernels' arithmetic (npde_arith) for the following steps of this handle. */
 npde_status npde_set_arithmetic(npde_solver* s, int arith);
 npde_status npde_get_arithmetic(const npde_solver* s, int* arith);
+/* The task layout (npde_task_layout) the current tuning gives this handle's instances. */
+npde_status npde_get_task_layout(const npde_solver* s, int* layout);
 /* Average device milliseconds of the stage-1 and stage-2 kernel launches over nsteps host-driven
  * steps of the current state (HIP events on the handle's stream around each launch; the state advances). */
 npde_status npde_time_stages(npde_solver* s, int nsteps, double* ms_stage1, double* ms_stage2);

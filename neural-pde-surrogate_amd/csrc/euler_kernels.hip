@@ -16,9 +16,10 @@
 
 namespace npde {
 
-hipError_t launch_stage_fast(int scheme, int lim, int stage, bool corr, int waves_per_block, int row_waves,
-                             const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks, Partial* part, int* ndone,
-                             hipStream_t s);
+hipError_t launch_stage_fast(int scheme, int lim, int stage, bool corr, int waves_per_block, int mode, int row_waves,
+                             int compact, int seam_rows, const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks,
+                             Partial* part, int* ndone, hipStream_t s);
+
 
 // ---------------------------------------------------------------------------------------------
 // Run-state machine of one instance (executed by one lane).  Mirrors the driver loop
@@ -122,13 +123,15 @@ __global__ __launch_bounds__(64) void k_finish(const InstDesc* __restrict__ desc
 }
 
 // Max wave speeds and first non-physical cell of U for every instance that is not done.  Same task
-// list, column mapping and partial slots as the stage kernels (strips or row groups).
-template <bool ROWS>
+// list, column mapping and partial slots as the stage kernels (strips, row groups or seams).
+template <int MODE>
 __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_wavespeed(const InstDesc* __restrict__ desc,
                                                                     Ctl* __restrict__ ctl,
                                                                     const Task* __restrict__ tasks, int ntasks,
                                                                     Partial* __restrict__ part,
                                                                     int* __restrict__ ndone) {
+    constexpr bool ROWS = MODE == kRowGroups;
+    constexpr bool SEAMS = MODE == kSeams;
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int nw = blockDim.x / kWave;
@@ -142,9 +145,9 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_wavespeed(const InstDe
     const InstDesc& D = desc[tk.inst];
     const int nx = D.nx;
     const long long P = D.plane;
-    const int i = ROWS ? wv * kWave + lane : tk.i0 - kHalo + lane;
-    const int ci = ROWS ? i : remap(i, nx, D.bc_x);
-    const bool owner = ROWS || (lane >= kHalo && lane < kWave - kHalo && i < nx);
+    const int i = ROWS ? wv * kWave + lane : (SEAMS ? tk.i0 + lane : tk.i0 - kHalo + lane);
+    const int ci = (ROWS || SEAMS) ? i : remap(i, nx, D.bc_x);
+    const bool owner = ROWS || SEAMS || (lane >= kHalo && lane < kWave - kHalo && i < nx);
     double sx = 0.0, sy = 0.0;
     int bad = kNoBad;
     const int ny = D.ny;
@@ -277,29 +280,41 @@ static inline void launch_dims(int ntasks, int waves_per_block, int row_waves, d
     }
 }
 
-hipError_t launch_stage(int scheme, int lim, int stage, bool corr, bool fast, int waves_per_block, int row_waves,
-                        const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks, Partial* part, int* ndone,
-                        hipStream_t s) {
-    if (row_waves < 0 || row_waves > kMaxRowWaves) return hipErrorInvalidValue;
+// mode: StageMode; row_waves: wavefronts per workgroup in row-group mode (1 .. kMaxRowWaves)
+static inline bool valid_mode(int mode, int row_waves) {
+    if (mode == kRowGroups) return row_waves >= 1 && row_waves <= kMaxRowWaves;
+    return (mode == kStrips || mode == kSeams) && row_waves == 0;
+}
+
+// seam mode: compact = read the stage input's edge columns from the edge arrays (the previous stage
+// kernel of this launch sequence wrote them); seam_rows = the band height (at most kMaxSeamRows), which
+// sizes each wavefront's LDS rows
+hipError_t launch_stage(int scheme, int lim, int stage, bool corr, bool fast, int waves_per_block, int mode,
+                        int row_waves, int compact, int seam_rows, const InstDesc* desc, Ctl* ctl, const Task* tasks,
+                        int ntasks, Partial* part, int* ndone, hipStream_t s) {
+    if (!valid_mode(mode, row_waves)) return hipErrorInvalidValue;
+    if (mode == kSeams && (seam_rows < 1 || seam_rows > kMaxSeamRows)) return hipErrorInvalidValue;
     if (fast)
-        return launch_stage_fast(scheme, lim, stage, corr, waves_per_block, row_waves, desc, ctl, tasks, ntasks, part,
-                                 ndone, s);
+        return launch_stage_fast(scheme, lim, stage, corr, waves_per_block, mode, row_waves, compact, seam_rows, desc,
+                                 ctl, tasks, ntasks, part, ndone, s);
     dim3 grid, block;
     launch_dims(ntasks, waves_per_block, row_waves, grid, block);
-    hipLaunchKernelGGL(pick_stage(scheme, lim, stage, corr, row_waves), grid, block, 0, s, desc, ctl, tasks, ntasks,
-                       part, ndone);
+    hipLaunchKernelGGL(pick_stage(scheme, lim, stage, corr, mode), grid, block, seam_lds_bytes(mode, waves_per_block, seam_rows),
+                       s, desc, ctl, tasks, ntasks, part, ndone, compact, seam_rows);
     return hipGetLastError();
 }
 
-hipError_t launch_wavespeed(int waves_per_block, int row_waves, const InstDesc* desc, Ctl* ctl, const Task* tasks,
-                            int ntasks, Partial* part, int* ndone, hipStream_t s) {
-    if (row_waves < 0 || row_waves > kMaxRowWaves) return hipErrorInvalidValue;
+hipError_t launch_wavespeed(int waves_per_block, int mode, int row_waves, const InstDesc* desc, Ctl* ctl,
+                            const Task* tasks, int ntasks, Partial* part, int* ndone, hipStream_t s) {
+    if (!valid_mode(mode, row_waves)) return hipErrorInvalidValue;
     dim3 grid, block;
     launch_dims(ntasks, waves_per_block, row_waves, grid, block);
-    if (row_waves > 0)
-        hipLaunchKernelGGL(k_wavespeed<true>, grid, block, 0, s, desc, ctl, tasks, ntasks, part, ndone);
+    if (mode == kRowGroups)
+        hipLaunchKernelGGL(k_wavespeed<kRowGroups>, grid, block, 0, s, desc, ctl, tasks, ntasks, part, ndone);
+    else if (mode == kSeams)
+        hipLaunchKernelGGL(k_wavespeed<kSeams>, grid, block, 0, s, desc, ctl, tasks, ntasks, part, ndone);
     else
-        hipLaunchKernelGGL(k_wavespeed<false>, grid, block, 0, s, desc, ctl, tasks, ntasks, part, ndone);
+        hipLaunchKernelGGL(k_wavespeed<kStrips>, grid, block, 0, s, desc, ctl, tasks, ntasks, part, ndone);
     return hipGetLastError();
 }
 

@@ -8,14 +8,15 @@
 
 namespace npde {
 
-hipError_t launch_stage_fast(int scheme, int lim, int stage, bool corr, int waves_per_block, int row_waves,
-                             const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks, Partial* part, int* ndone,
-                             hipStream_t s) {
+hipError_t launch_stage_fast(int scheme, int lim, int stage, bool corr, int waves_per_block, int mode, int row_waves,
+                             int compact, int seam_rows, const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks,
+                             Partial* part, int* ndone, hipStream_t s) {
     const dim3 grid = row_waves > 0 ? dim3(unsigned(ntasks))
                                     : dim3(unsigned((ntasks + waves_per_block - 1) / waves_per_block));
     const dim3 block = dim3(unsigned((row_waves > 0 ? row_waves : waves_per_block) * kWave));
-    hipLaunchKernelGGL(pick_stage(scheme, lim, stage, corr, row_waves), grid, block, 0, s, desc, ctl, tasks, ntasks,
-                       part, ndone);
+    hipLaunchKernelGGL(pick_stage(scheme, lim, stage, corr, mode), grid, block,
+                       seam_lds_bytes(mode, waves_per_block, seam_rows), s, desc, ctl, tasks, ntasks, part, ndone,
+                       compact, seam_rows);
     return hipGetLastError();
 }
 

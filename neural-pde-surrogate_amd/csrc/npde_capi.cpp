@@ -22,11 +22,11 @@
 #include "npde_device.hpp"
 
 namespace npde {
-hipError_t launch_stage(int scheme, int lim, int stage, bool corr, bool fast, int waves_per_block, int row_waves,
-                        const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks, Partial* part, int* ndone,
-                        hipStream_t s);
-hipError_t launch_wavespeed(int waves_per_block, int row_waves, const InstDesc* desc, Ctl* ctl, const Task* tasks,
-                            int ntasks, Partial* part, int* ndone, hipStream_t s);
+hipError_t launch_stage(int scheme, int lim, int stage, bool corr, bool fast, int waves_per_block, int mode,
+                        int row_waves, int compact, int seam_rows, const InstDesc* desc, Ctl* ctl, const Task* tasks,
+                        int ntasks, Partial* part, int* ndone, hipStream_t s);
+hipError_t launch_wavespeed(int waves_per_block, int mode, int row_waves, const InstDesc* desc, Ctl* ctl,
+                            const Task* tasks, int ntasks, Partial* part, int* ndone, hipStream_t s);
 hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int nftasks, int* ndone,
                         hipStream_t s);
 hipError_t launch_block_average(const InstDesc* desc, int inst, int m, int nyc, float* out, hipStream_t s);
@@ -53,6 +53,8 @@ struct npde_solver {
     InstDesc* ddesc = nullptr;
     Ctl* dctl = nullptr;
     Task* dtasks = nullptr;
+    double* dedge = nullptr;  // edge columns of every instance (seam mode)
+    size_t edge_cap = 0;      // doubles allocated at dedge
     FTask* dftasks = nullptr;
     Partial* dpart = nullptr;
     int* dndone = nullptr;
@@ -67,8 +69,10 @@ struct npde_solver {
     bool fast = false;  // NPDE_ARITH_FAST: the fast-arithmetic stage kernels
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    npde_tuning tune{0, 4, 32, 1, 1};
-    int row_waves = 0;  // > 0: the task list is in row-group form with this many wavefronts per band
+    npde_tuning tune{0, 4, 32, 1, 1, 1};
+    int mode = kStrips;  // StageMode of the current task list
+    int row_waves = 0;   // row groups: wavefronts per band
+    int band = 0;        // band height of the current task list
     hipGraphExec_t graph = nullptr;
     int graph_key_frames = -1;  // nftasks the cached graph was captured with (0: no frame kernel)
     int graph_key_steps = 0;
@@ -108,14 +112,10 @@ bool valid_grid(const npde_grid_desc& g, std::string& why) {
 
 // Band height: the tuned value, or (rows_per_task <= 0) the tallest band in [4, 64] that still gives
 // about 4096 tasks, so that small grids are not bound by one wavefront marching many rows.
-int band_rows(const npde_solver* s) {
+int band_rows(const npde_solver* s, int strip_cols) {
     if (s->tune.rows_per_task > 0) return s->tune.rows_per_task;
-    long long cols = 0, rows_total = 0;
-    for (const auto& g : s->grids) {
-        cols += (g.nx + kStripCols - 1) / kStripCols;
-        rows_total += (long long)((g.nx + kStripCols - 1) / kStripCols) * g.ny;
-    }
-    (void)cols;
+    long long rows_total = 0;
+    for (const auto& g : s->grids) rows_total += (long long)((g.nx + strip_cols - 1) / strip_cols) * g.ny;
     int r = 64;
     while (r > 4 && rows_total / r < 4096) r /= 2;
     return r;
@@ -137,13 +137,29 @@ int row_group_waves_for(const npde_solver* s, const npde_tuning& tune) {
 }
 int row_group_waves(const npde_solver* s) { return row_group_waves_for(s, s->tune); }
 
-// Task list and partial slots.  Strips: one task and one partial per wavefront.  Row groups: one task
-// per band (workgroup), W partials per task; D.task0 / D.ntasks index partial slots in both forms.
+// Seam mode applies when every instance's width is a multiple of 64, row groups do not apply and bands
+// are at most 64 rows (each wavefront keeps its band's seam solves in LDS): each wavefront then owns a
+// 64-column strip and solves the two strip boundaries of its band itself before its march, so no halo
+// column is recomputed and no wavefront waits for another.
+bool seams_apply_for(const npde_solver* s, const npde_tuning& tune) {
+    if (tune.seams <= 0 || s->n == 0 || row_group_waves_for(s, tune) > 0) return false;
+    if (tune.rows_per_task > 64) return false;
+    for (const auto& g : s->grids)
+        if (g.nx % kWave != 0) return false;
+    return true;
+}
+
+// Task list and partial slots.  Strips and seams: one task and one partial per wavefront.  Row groups:
+// one task per band (workgroup), W partials per task; D.task0 / D.ntasks index partial slots in every
+// form.
 void build_tasks(npde_solver* s) {
     s->htasks.clear();
     s->row_waves = row_group_waves(s);
+    s->mode = s->row_waves > 0 ? kRowGroups : (seams_apply_for(s, s->tune) ? kSeams : kStrips);
     const int W = s->row_waves;
-    const int rows = band_rows(s);
+    const int cols = s->mode == kStrips ? kStripCols : kWave;
+    const int rows = band_rows(s, cols);
+    s->band = rows;
     for (int k = 0; k < s->n; ++k) {
         InstDesc& D = s->hdesc[k];
         const int first = int(s->htasks.size());
@@ -152,10 +168,9 @@ void build_tasks(npde_solver* s) {
             D.task0 = first * W;
             D.ntasks = (int(s->htasks.size()) - first) * W;
         } else {
-            const int strips = (D.nx + kStripCols - 1) / kStripCols;
+            const int strips = (D.nx + cols - 1) / cols;
             for (int j0 = 0; j0 < D.ny; j0 += rows)
-                for (int st = 0; st < strips; ++st)
-                    s->htasks.push_back({k, st * kStripCols, j0, std::min(j0 + rows, D.ny)});
+                for (int st = 0; st < strips; ++st) s->htasks.push_back({k, st * cols, j0, std::min(j0 + rows, D.ny)});
             D.task0 = first;
             D.ntasks = int(s->htasks.size()) - first;
         }
@@ -170,6 +185,27 @@ npde_status upload_tasks(npde_solver* s) {
     if (s->dpart) (void)hipFree(s->dpart);
     s->dtasks = nullptr;
     s->dpart = nullptr;
+    // seam mode, per instance: the edge columns of U and of U1, each a left and a right array of
+    // (nx/64) x ny x kEdgeRecord doubles
+    auto edge_doubles = [](const InstDesc& D) { return size_t(2 * kEdgeRecord) * (D.nx / kWave) * D.ny; };
+    size_t edge_need = 0;
+    if (s->mode == kSeams)
+        for (const auto& D : s->hdesc) edge_need += 2 * edge_doubles(D);
+    if (edge_need > s->edge_cap) {
+        if (s->dedge) (void)hipFree(s->dedge);
+        s->dedge = nullptr;
+        s->edge_cap = 0;
+        HIPCHK(s, hipMalloc(&s->dedge, edge_need * sizeof(double)));
+        s->edge_cap = edge_need;
+    }
+    size_t off = 0;
+    for (auto& D : s->hdesc) {
+        D.edge[0] = D.edge[1] = nullptr;
+        if (s->mode != kSeams) continue;
+        D.edge[0] = s->dedge + off;
+        D.edge[1] = D.edge[0] + edge_doubles(D);
+        off += 2 * edge_doubles(D);
+    }
     HIPCHK(s, hipMalloc(&s->dtasks, sizeof(Task) * s->htasks.size()));
     HIPCHK(s, hipMalloc(&s->dpart, sizeof(Partial) * partial_slots(s)));
     HIPCHK(s, hipMemcpy(s->dtasks, s->htasks.data(), sizeof(Task) * s->htasks.size(), hipMemcpyHostToDevice));
@@ -217,20 +253,31 @@ npde_status fill_bad(npde_solver* s, npde_bad_cell* bad) {
     return NPDE_OK;
 }
 
-// One RK2 step of every stepping instance: two stage launches, then the per-instance reduction of
-// the fused wave-speed partials and the run-state decision.
-npde_status launch_step_pair(npde_solver* s) {
-    HIPCHK(s, launch_stage(s->flux, s->lim, 1, s->has_corr, s->fast, s->tune.waves_per_block, s->row_waves, s->ddesc, s->dctl, s->dtasks,
-                           int(s->htasks.size()), s->dpart, s->dndone, s->stream));
-    HIPCHK(s, launch_stage(s->flux, s->lim, 2, s->has_corr, s->fast, s->tune.waves_per_block, s->row_waves, s->ddesc, s->dctl, s->dtasks,
-                           int(s->htasks.size()), s->dpart, s->dndone, s->stream));
+// One stage launch.  In seam mode the wavefronts' seam solves read the stage input's edge columns from
+// the edge arrays, except in stage 1 of the first step of a launch sequence (first_step: a host call, a
+// graph, an iteration loop), whose input the caller may have changed since a stage 2 last wrote them.
+npde_status launch_stage_kernel(npde_solver* s, int stage, bool first_step) {
+    const int compact = (stage == 2 || !first_step) ? 1 : 0;
+    HIPCHK(s, launch_stage(s->flux, s->lim, stage, s->has_corr, s->fast, s->tune.waves_per_block, s->mode,
+                           s->row_waves, compact, s->band, s->ddesc, s->dctl, s->dtasks, int(s->htasks.size()),
+                           s->dpart, s->dndone, s->stream));
+    return NPDE_OK;
+}
+
+// One RK2 step of every stepping instance: two stage launches, then the per-instance reduction of the
+// fused wave-speed partials and the run-state decision.
+npde_status launch_step_pair(npde_solver* s, bool first_step) {
+    for (int stage = 1; stage <= 2; ++stage) {
+        npde_status st = launch_stage_kernel(s, stage, first_step);
+        if (st != NPDE_OK) return st;
+    }
     HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterStep, s->dndone, s->stream));
     return NPDE_OK;
 }
 
 npde_status launch_wavespeed_and_finish(npde_solver* s) {
-    HIPCHK(s, launch_wavespeed(s->tune.waves_per_block, s->row_waves, s->ddesc, s->dctl, s->dtasks, int(s->htasks.size()),
-                               s->dpart, s->dndone, s->stream));
+    HIPCHK(s, launch_wavespeed(s->tune.waves_per_block, s->mode, s->row_waves, s->ddesc, s->dctl, s->dtasks,
+                               int(s->htasks.size()), s->dpart, s->dndone, s->stream));
     HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterWavespeed, s->dndone, s->stream));
     return NPDE_OK;
 }
@@ -241,7 +288,7 @@ npde_status launch_iterations(npde_solver* s, int iters, int nft) {
             HIPCHK(s, launch_frame(s->ddesc, s->dctl, s->dftasks, nft, s->dndone, s->stream));
             HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterFrame, s->dndone, s->stream));
         }
-        npde_status st = launch_step_pair(s);
+        npde_status st = launch_step_pair(s, it == 0);
         if (st != NPDE_OK) return st;
     }
     return NPDE_OK;
@@ -440,8 +487,9 @@ npde_status npde_destroy(npde_solver* s) {
     if (!s) return NPDE_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     drop_graph(s);
-    for (void* p : {(void*)s->ddesc, (void*)s->dctl, (void*)s->dtasks, (void*)s->dftasks, (void*)s->dpart,
-                    (void*)s->dndone, (void*)s->ddts, (void*)s->dstate, (void*)s->dtraj, (void*)s->dftimes})
+    for (void* p : {(void*)s->ddesc, (void*)s->dctl, (void*)s->dtasks, (void*)s->dedge,
+                    (void*)s->dftasks, (void*)s->dpart, (void*)s->dndone, (void*)s->ddts, (void*)s->dstate,
+                    (void*)s->dtraj, (void*)s->dftimes})
         if (p) (void)hipFree(p);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -585,14 +633,16 @@ npde_status npde_compute_dt(npde_solver* s, double* dt, npde_bad_cell* bad) {
     return fill_bad(s, bad);
 }
 
-npde_status npde_step(npde_solver* s, const double* dt) {
+// One host-driven step with the given dts, timed into s->last_ms.  first_step: the first of a sequence
+// of such steps (the caller may have changed the state before it).
+static npde_status host_step(npde_solver* s, const double* dt, bool first_step) {
     if (!s || !dt) return fail(s, NPDE_ERR_INVALID, "null argument");
     npde_status st = set_host_mode(s);
     if (st != NPDE_OK) return st;
     HIPCHK(s, hipMemcpyAsync(s->ddts, dt, sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
     HIPCHK(s, launch_prep_host_step(s->dctl, s->n, s->ddts, s->stream));
     HIPCHK(s, hipEventRecord(s->ev0, s->stream));
-    if ((st = launch_step_pair(s)) != NPDE_OK) return st;
+    if ((st = launch_step_pair(s, first_step)) != NPDE_OK) return st;
     HIPCHK(s, hipEventRecord(s->ev1, s->stream));
     HIPCHK(s, hipEventSynchronize(s->ev1));
     float ms = 0.f;
@@ -600,6 +650,8 @@ npde_status npde_step(npde_solver* s, const double* dt) {
     s->last_ms = ms;
     return NPDE_OK;
 }
+
+npde_status npde_step(npde_solver* s, const double* dt) { return host_step(s, dt, true); }
 
 npde_status npde_advance_to(npde_solver* s, double t_target, npde_bad_cell* bad) {
     if (!s) return NPDE_ERR_INVALID;
@@ -618,30 +670,35 @@ npde_status npde_time_stages(npde_solver* s, int nsteps, double* ms_stage1, doub
     if (st != NPDE_OK) return st;
     if ((st = set_host_mode(s)) != NPDE_OK) return st;
     HIPCHK(s, hipMemcpyAsync(s->ddts, dts.data(), sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
-    hipEvent_t e[3];
-    for (auto& x : e) HIPCHK(s, hipEventCreate(&x));
-    double t1 = 0.0, t2 = 0.0;
-    for (int k = 0; k < nsteps && st == NPDE_OK; ++k) {
-        HIPCHK(s, launch_prep_host_step(s->dctl, s->n, s->ddts, s->stream));
-        HIPCHK(s, hipEventRecord(e[0], s->stream));
-        HIPCHK(s, launch_stage(s->flux, s->lim, 1, s->has_corr, s->fast, s->tune.waves_per_block, s->row_waves, s->ddesc, s->dctl,
-                               s->dtasks, int(s->htasks.size()), s->dpart, s->dndone, s->stream));
-        HIPCHK(s, hipEventRecord(e[1], s->stream));
-        HIPCHK(s, launch_stage(s->flux, s->lim, 2, s->has_corr, s->fast, s->tune.waves_per_block, s->row_waves, s->ddesc, s->dctl,
-                               s->dtasks, int(s->htasks.size()), s->dpart, s->dndone, s->stream));
-        HIPCHK(s, hipEventRecord(e[2], s->stream));
-        HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterStep, s->dndone, s->stream));
-        HIPCHK(s, hipEventSynchronize(e[2]));
-        float a = 0.f, b = 0.f;
-        HIPCHK(s, hipEventElapsedTime(&a, e[0], e[1]));
-        HIPCHK(s, hipEventElapsedTime(&b, e[1], e[2]));
-        t1 += a;
-        t2 += b;
+    hipEvent_t e[3] = {};
+    hipError_t he = hipSuccess;
+    for (auto& x : e)
+        if (he == hipSuccess) he = hipEventCreate(&x);
+    double acc[2] = {0.0, 0.0};
+    // the steps follow each other as in the device loop (the first opens the sequence)
+    for (int k = 0; k < nsteps && st == NPDE_OK && he == hipSuccess; ++k) {
+        he = launch_prep_host_step(s->dctl, s->n, s->ddts, s->stream);
+        for (int stage = 1; stage <= 2 && st == NPDE_OK && he == hipSuccess; ++stage) {
+            he = hipEventRecord(e[stage - 1], s->stream);
+            if (he == hipSuccess) st = launch_stage_kernel(s, stage, k == 0);
+        }
+        if (st == NPDE_OK && he == hipSuccess) he = hipEventRecord(e[2], s->stream);
+        if (st == NPDE_OK && he == hipSuccess)
+            he = launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterStep, s->dndone, s->stream);
+        if (st == NPDE_OK && he == hipSuccess) he = hipEventSynchronize(e[2]);
+        for (int q = 0; q < 2 && st == NPDE_OK && he == hipSuccess; ++q) {
+            float m = 0.f;
+            he = hipEventElapsedTime(&m, e[q], e[q + 1]);
+            acc[q] += m;
+        }
     }
-    for (auto& x : e) (void)hipEventDestroy(x);
-    *ms_stage1 = t1 / nsteps;
-    *ms_stage2 = t2 / nsteps;
-    return st;
+    for (auto& x : e)
+        if (x) (void)hipEventDestroy(x);
+    if (st != NPDE_OK) return st;
+    if (he != hipSuccess) return fail(s, NPDE_ERR_DEVICE, std::string("timing: ") + hipGetErrorString(he));
+    *ms_stage1 = acc[0] / nsteps;
+    *ms_stage2 = acc[1] / nsteps;
+    return NPDE_OK;
 }
 
 npde_status npde_run_trajectory(npde_solver* s, double t_start, double t_end, int nout, const int* sizes, int n_sizes,
@@ -782,7 +839,8 @@ npde_status npde_set_tuning(npde_solver* s, const npde_tuning* t) {
     if (t->rows_per_task < 0 || !(t->waves_per_block == 1 || t->waves_per_block == 2 || t->waves_per_block == 4) ||
         t->steps_per_graph < 1)
         return fail(s, NPDE_ERR_INVALID, "bad tuning");
-    const bool layout_changed = t->rows_per_task != s->tune.rows_per_task || t->row_groups != s->tune.row_groups;
+    const bool layout_changed = t->rows_per_task != s->tune.rows_per_task || t->row_groups != s->tune.row_groups ||
+                                t->seams != s->tune.seams;
     s->tune = *t;
     drop_graph(s);
     if (layout_changed) return upload_tasks(s);
@@ -800,6 +858,15 @@ npde_status npde_set_arithmetic(npde_solver* s, int arith) {
 npde_status npde_get_arithmetic(const npde_solver* s, int* arith) {
     if (!s || !arith) return NPDE_ERR_INVALID;
     *arith = s->fast ? NPDE_ARITH_FAST : NPDE_ARITH_EXACT;
+    return NPDE_OK;
+}
+
+npde_status npde_get_task_layout(const npde_solver* s, int* layout) {
+    if (!s || !layout) return NPDE_ERR_INVALID;
+    static_assert(int(NPDE_TASKS_STRIPS) == kStrips && int(NPDE_TASKS_ROW_GROUPS) == kRowGroups &&
+                      int(NPDE_TASKS_SEAMS) == kSeams,
+                  "npde_task_layout mirrors StageMode");
+    *layout = s->mode;
     return NPDE_OK;
 }
 
@@ -898,7 +965,7 @@ npde_status npde_builtin_problem(const char* test, int n, npde_grid_desc* grid, 
     return NPDE_OK;
 }
 
-// Autotuner: times candidate (rows_per_task, waves_per_block) pairs on the handle's own instances
+// Autotuner: times candidate (task layout, rows_per_task, waves_per_block) on the handle's own instances
 // with host-driven steps of the current CFL dt, restoring the state after every trial.  Every
 // candidate computes the same bits (the per-cell arithmetic does not depend on the launch shape);
 // only the time differs.
@@ -915,12 +982,13 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
     if (cache_path) {
         std::ifstream in(cache_path);
         std::string line;
-        while (std::getline(in, line)) {  // "key rows waves_per_block [row_groups]"
+        while (std::getline(in, line)) {  // "key rows waves_per_block [row_groups [seams]]"
             std::istringstream ls(line);
             std::string k;
             npde_tuning t = s->tune;
             if (!(ls >> k >> t.rows_per_task >> t.waves_per_block) || k != key.str()) continue;
             if (!(ls >> t.row_groups)) t.row_groups = 0;
+            if (!(ls >> t.seams)) t.seams = 0;
             npde_status st = npde_set_tuning(s, &t);
             if (st == NPDE_OK && chosen) *chosen = s->tune;
             return st;
@@ -952,25 +1020,36 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
     for (const auto& g : s->grids) max_ny = std::max(max_ny, g.ny);
     const int rows_c[] = {4, 8, 16, 32, 64, 128};
     const int wpb_c[] = {1, 2, 4};
+    // task layouts that apply to this batch, as (row_groups, seams): row groups up to W = 8, seam
+    // strips, plain strips
     npde_tuning probe = orig;
     probe.row_groups = 2;
-    const bool rows_ok = row_group_waves_for(s, probe) > 0;  // row groups apply to this batch
-    for (int rg = rows_ok ? 2 : 0; rg >= 0; rg -= 2)
+    std::vector<std::pair<int, int>> layouts;
+    if (row_group_waves_for(s, probe) > 0) layouts.push_back({2, 0});
+    probe.row_groups = 0;
+    probe.seams = 1;
+    if (seams_apply_for(s, probe)) layouts.push_back({0, 1});
+    layouts.push_back({0, 0});
+    for (const auto& lay : layouts)
     for (int r : rows_c) {
         if (st != NPDE_OK) break;
         if (r > max_ny && r != rows_c[0]) continue;
+        if (lay.second && r > 64) continue;  // seam bands keep their seam solves in LDS: at most 64 rows
         for (int w : wpb_c) {
-            if (rg && w != wpb_c[0]) continue;  // a row group's workgroup size is fixed by the width
+            if (lay.first && w != wpb_c[0]) continue;  // a row group's workgroup size is fixed by the width
             npde_tuning t = orig;
             t.rows_per_task = r;
             t.waves_per_block = w;
-            t.row_groups = rg;
+            t.row_groups = lay.first;
+            t.seams = lay.second;
             if ((st = npde_set_tuning(s, &t)) != NPDE_OK) break;
-            // the fastest of 5 timed steps (after one warm-up): robust to a stray slow step
+            // the fastest of 5 timed steps (after one warm-up), each the second step of a sequence as in
+            // the device loop: robust to a stray slow step
             double ms_min = 1e30;
             for (int rep = 0; rep < 6 && st == NPDE_OK; ++rep) {
                 copy_state(false);
-                st = npde_step(s, dts.data());
+                st = host_step(s, dts.data(), true);
+                if (st == NPDE_OK) st = host_step(s, dts.data(), false);
                 if (rep > 0) ms_min = std::min(ms_min, s->last_ms);
             }
             if (st == NPDE_OK && ms_min < best_ms) {
@@ -993,7 +1072,7 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
     if (st == NPDE_OK && cache_path) {
         std::ofstream out(cache_path, std::ios::app);
         out << key.str() << " " << best.rows_per_task << " " << best.waves_per_block << " " << best.row_groups
-            << "\n";
+            << " " << best.seams << "\n";
     }
     if (chosen) *chosen = s->tune;
     return st;

@@ -6,11 +6,15 @@
 // reference's ghost fill would copy, grid2d.hpp:43-66).  Instances of a batch live back to back in
 // one allocation, so a batch of 64 x 512^2 is 1.07 GB of U + U1.
 //
-// Work decomposition: a "task" is one wavefront's share of one RK stage -- a strip of 60 interior
-// columns (64 lanes minus a 2-column halo on each side) and a band of rows, marched upward row by
-// row.  Tasks of all instances form one list; a stage launch covers the whole list, and waves of
-// instances that are not stepping leave at once.  In row-group mode (periodic x, width 64*W) a task
-// is a band of all the columns, run by one workgroup of W wavefronts with one partial per wavefront.
+// Work decomposition: a "task" is one wavefront's share of one RK stage -- a strip of columns and a
+// band of rows, marched upward row by row.  Tasks of all instances form one list; a stage launch
+// covers the whole list, and waves of instances that are not stepping leave at once.  Three forms
+// (StageMode):
+//   strips     60 interior columns per wavefront (64 lanes minus a 2-column halo on each side);
+//   row groups periodic x, width 64*W: a task is a band of all the columns, run by one workgroup of
+//              W wavefronts that exchange edge columns through LDS, one partial per wavefront;
+//   seams      width a multiple of 64: 64 owned columns per wavefront; each wavefront first solves the
+//              two strip boundaries ("seams") of its band, from edge columns the previous stage wrote.
 #pragma once
 #include <cstdint>
 
@@ -24,6 +28,14 @@ constexpr int kMaxLevels = 4;                  // trajectory resolutions per run
 constexpr int kNoBad = 0x7fffffff;             // "no non-physical cell" marker
 
 enum Mode : int { kModeHost = 0, kModeAuto = 1 };
+enum StageMode : int { kStrips = 0, kRowGroups = 1, kSeams = 2 };
+
+// Edge columns (seam mode): the stage kernels also write the conserved values of columns 0, 1 (left
+// edge) and 62, 63 (right edge) of every 64-column strip, so that the seam solves of the next stage read
+// them contiguously instead of gathering four columns per row from the full state.  Per state buffer
+// (U or U1) and instance: a left array and a right array, each (nx/64) x ny x 8 doubles; column e of
+// the pair (0 or 1), variable q at [(strip*ny + j)*8 + e*4 + q].
+constexpr int kEdgeRecord = 8;
 
 struct InstDesc {
     double* U;        // 4 planes: state (stage-1 stencil input, stage-2 output)
@@ -35,6 +47,7 @@ struct InstDesc {
     int ftask0, nftasks;         // this instance's slice of the frame task list
     const double* corr_x;        // optional learned x-face flux increments (4 planes), or null
     const double* corr_y;        // optional learned y-face flux increments (4 planes), or null
+    double* edge[2];             // seam mode: edge columns of U [0] and U1 [1] (left array, then right), else null
     int nlevels;
     int lvl_m[kMaxLevels];       // coarse size (columns) of each trajectory level
     int lvl_f[kMaxLevels];       // coarsening factor nx / m

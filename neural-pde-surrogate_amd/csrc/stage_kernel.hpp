@@ -19,6 +19,15 @@
 //     a few LDS slots, read as the `old` operand of the DPP shift, so no halo column is recomputed;
 //     two workgroup barriers per row order the exchanges (the host uses it up to W = 4 by default:
 //     an 8-wavefront workgroup at these register counts is alone on its CU and waits at each barrier);
+//   * seam mode (SEAMS): for widths that are a multiple of 64, every lane owns a column of a 64-column
+//     strip.  Before its march, the wavefront solves the two strip boundaries ("seams") of its band
+//     in parallel (one lane per seam and row) into LDS: the primitives of the outside neighbour
+//     column and the x flux through the seam.  During the march lanes 0 and 63 take the neighbour's
+//     primitives as the `old` operand of the DPP shift, lane 0 its left-face flux and lane 63 its
+//     right-face flux from there.  The seam solves read the four columns around each seam from
+//     small edge arrays the previous stage kernel wrote (its lanes 0, 1, 62, 63), not from the full
+//     state.  The same functions on the same operands produce every value, so the bits are those of
+//     the other modes, and no wavefront waits for another;
 //   * workgroups are renumbered so that each XCD runs a contiguous range of the task list, which
 //     keeps neighbouring tasks (overlapping halos) in one L2;
 //   * the row loop is unrolled so that the carried rows are renamed instead of copied;
@@ -118,6 +127,9 @@ __device__ __forceinline__ int wave_min(int x) {
 // lane offset) instead of flat accesses.
 typedef __attribute__((address_space(1))) const double gcdouble;
 typedef __attribute__((address_space(1))) double gdouble;
+struct alignas(16) DPair { double a, b; };  // 16-byte loads and stores of edge-array entries
+typedef __attribute__((address_space(1))) const DPair gcpair;
+typedef __attribute__((address_space(1))) DPair gpair;
 
 __device__ __forceinline__ Cons load4(const double* B, long long off, long long P) {
     const gcdouble* g = (const gcdouble*)B;
@@ -158,12 +170,59 @@ __device__ __forceinline__ void publish_partial(Partial* __restrict__ part, int 
 }
 
 // ---------------------------------------------------------------------------------------------
-template <int SCHEME, int LIM, int STAGE, bool CORR, bool ROWS>
+// Seam mode: the face left of column i0 (a multiple of 64, 0 <= i0 <= nx) in row j, solved as the strip
+// lanes beside it would solve it: the primitives of columns i0-2 .. i0+1 (ghosts remapped), the limited
+// slopes of columns i0-1 and i0, their facing edge sides, the Riemann solve and the face's learned
+// increment.  The four cells come from the edge arrays E written by the previous stage kernel
+// (compact), or from the state V itself.  Gives the primitives of the cells left (wl) and right (wr) of
+// the face and its flux F.
+template <int SCHEME, int LIM>
+__device__ __forceinline__ void seam_face(const InstDesc& D, const double* V, const double* E, bool compact,
+                                          bool corr, int i0, int j, Prim& wl, Prim& wr, Cons& F) {
+    const int nx = D.nx, ny = D.ny;
+    const long long P = D.plane, r = (long long)j * nx;
+    const long long R = (long long)(nx / kWave) * ny * kEdgeRecord;  // right arrays follow the left ones
+    auto cell = [&](int i) -> Cons {
+        const int c = remap(i, nx, D.bc_x);
+        if (!compact) return load4(V, r + c, P);
+        const int st = c / kWave, e = c % kWave;  // e is 0, 1 (left edge) or 62, 63 (right edge)
+        const gcpair* p = (const gcpair*)(E + (e < 2 ? 0 : R) + ((long long)st * ny + j) * kEdgeRecord + (e & 1) * 4);
+        return {p[0].a, p[0].b, p[1].a, p[1].b};
+    };
+    const Prim w0 = prim_of(cell(i0 - 2));
+    wl = prim_of(cell(i0 - 1));
+    wr = prim_of(cell(i0));
+    const Prim w3 = prim_of(cell(i0 + 1));
+    const Side rt = edge_side(wl, slope<LIM>(w0, wl, wr), +0.5);
+    const Side lf = edge_side(wr, slope<LIM>(wl, wr, w3), -0.5);
+    F = flux_x<SCHEME>(rt, lf);
+    if (corr && D.corr_x != nullptr && D.corr_y != nullptr) {
+        // the increment of the face left of column i0 (the right boundary face wraps when periodic)
+        const int cxi = i0 < nx ? i0 : (D.bc_x == kPeriodic ? remap(i0, nx, kPeriodic) : -1);
+        if (cxi >= 0) {
+            const Cons k = load4(D.corr_x, r + cxi, P);
+            F = {F.d + k.d, F.m + k.m, F.n + k.n, F.e + k.e};
+        }
+    }
+}
+
+// seam mode: LDS doubles per wavefront and band row (for each of the strip's two seams, the outside
+// neighbour's primitives and the seam's flux); bands are at most kMaxSeamRows rows
+constexpr int kSeamLds = 16;
+constexpr int kMaxSeamRows = 64;
+inline unsigned seam_lds_bytes(int mode, int waves_per_block, int seam_rows) {
+    return mode == kSeams ? unsigned(waves_per_block * seam_rows * kSeamLds * sizeof(double)) : 0u;
+}
+
+template <int SCHEME, int LIM, int STAGE, bool CORR, int MODE>
 __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* __restrict__ desc,
                                                                 Ctl* __restrict__ ctl,
                                                                 const Task* __restrict__ tasks, int ntasks,
                                                                 Partial* __restrict__ part,
-                                                                int* __restrict__ ndone) {
+                                                                int* __restrict__ ndone, int compact,
+                                                                int seam_rows) {
+    constexpr bool ROWS = MODE == kRowGroups;
+    constexpr bool SEAMS = MODE == kSeams;
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wavefront in the workgroup
     const int nw = blockDim.x / kWave;
@@ -181,9 +240,17 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* 
     const long long P = D.plane;
     const double cx = D.cx, cy = D.cy;
 
-    const int i = ROWS ? wv * kWave + lane : tk.i0 - kHalo + lane;  // this lane's column
-    const int ci = ROWS ? i : remap(i, nx, D.bc_x);  // where it is loaded from (in range for every lane)
-    const bool owner = ROWS || (lane >= kHalo && lane < kWave - kHalo && i < nx);
+    // this lane's column, and where it is loaded from (in range for every lane)
+    const int i = ROWS ? wv * kWave + lane : (SEAMS ? tk.i0 + lane : tk.i0 - kHalo + lane);
+    const int ci = (ROWS || SEAMS) ? i : remap(i, nx, D.bc_x);
+    const bool owner = ROWS || SEAMS || (lane >= kHalo && lane < kWave - kHalo && i < nx);
+
+    gpair* eout = nullptr;  // seams, lanes 0, 1, 62, 63: where this lane's output column goes in the edge arrays
+    if constexpr (SEAMS) {
+        const long long strips = nx / kWave;
+        eout = (gpair*)(D.edge[STAGE == 1 ? 1 : 0] + (lane >= kWave / 2 ? strips * ny * kEdgeRecord : 0) +
+                        (long long)(tk.i0 / kWave) * ny * kEdgeRecord + (lane & 1) * 4);
+    }
 
     // row groups: edge values exchanged between the wavefronts of the workgroup.  s_w holds the edge
     // lanes' primitives of a row (double-buffered by row parity), s_rt lane 63's right-face side,
@@ -196,6 +263,31 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* 
 
     const double* __restrict__ V = (STAGE == 1) ? D.U : D.U1;  // stencil input
     double* O = (STAGE == 1) ? D.U1 : D.U;                     // output (stage 2: in place on U)
+
+    // seams: before the march, lanes 0-31 solve the seam left of the strip and lanes 32-63 the seam right
+    // of it, 32 band rows at a time, into this wavefront's LDS rows: the outside neighbour's primitives
+    // (column i0-1 for lane 0, i0+64 for lane 63) and the seam's flux.  Only this wavefront reads them,
+    // and a wavefront's LDS accesses complete in order, so no barrier is needed.
+    double* sseam = nullptr;
+    if constexpr (SEAMS) {
+        extern __shared__ double s_dyn[];
+        sseam = s_dyn + wv * seam_rows * kSeamLds;
+        const int side = lane >> 5;
+        for (int r0 = tk.j0; r0 < tk.j1; r0 += kWave / 2) {
+            const int j = r0 + (lane & (kWave / 2 - 1));
+            if (j < tk.j1) {
+                Prim wl, wr;
+                Cons F;
+                seam_face<SCHEME, LIM>(D, V, D.edge[STAGE == 1 ? 0 : 1], compact != 0, CORR, tk.i0 + side * kWave, j,
+                                       wl, wr, F);
+                const Prim& w = side ? wr : wl;
+                double* o = sseam + (j - tk.j0) * kSeamLds + side * (kSeamLds / 2);
+                o[0] = w.d; o[1] = w.u; o[2] = w.v; o[3] = w.p;
+                o[4] = F.d; o[5] = F.m; o[6] = F.n; o[7] = F.e;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
 
     auto row = [&](int j) -> long long { return (long long)remap(j, ny, bcy) * nx + ci; };
 
@@ -267,10 +359,15 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* 
 
         // x: slope of this cell from the neighbouring lanes, left-face flux here, right-face flux from lane+1
         Prim dxs;
-        if constexpr (ROWS)
+        if constexpr (ROWS) {
             dxs = slope<LIM>(from_prev_or(wa, s_w[f & 1][1][wl]), wa, from_next_or(wa, s_w[f & 1][0][wr]));
-        else
+        } else if constexpr (SEAMS) {
+            const double* q = sseam + (f - j0) * kSeamLds + (lane >> 5) * (kSeamLds / 2);
+            const double o[4] = {q[0], q[1], q[2], q[3]};
+            dxs = slope<LIM>(from_prev_or(wa, o), wa, from_next_or(wa, o));
+        } else {
             dxs = slope<LIM>(from_prev(wa), wa, from_next(wa));
+        }
         const Side lf = edge_side(wa, dxs, -0.5);
         const Side rt = edge_side(wa, dxs, +0.5);
         Side rl;  // right-face side of the column to the left
@@ -297,6 +394,12 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* 
             if (lane == kWave - 1) put(s_w[(f + 1) & 1][1][wv], wb);
             __syncthreads();
             fr = from_next_or(fl, s_fl[wr]);
+        } else if constexpr (SEAMS) {
+            // lane 0's left face is the left seam (its own Riemann solve had no left neighbour)
+            const double* q = sseam + (f - j0) * kSeamLds + (lane >> 5) * (kSeamLds / 2);
+            const double F[4] = {q[4], q[5], q[6], q[7]};
+            if (lane == 0) fl = {F[0], F[1], F[2], F[3]};
+            fr = from_next_or(fl, F);
         } else {
             fr = from_next(fl);
         }
@@ -320,6 +423,15 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* 
                    0.5 * (ub.n + va.n + dt * L.n), 0.5 * (ub.e + va.e + dt * L.e)};
         }
         if (owner) store4(O, (long long)f * nx + ci, P, out);
+        if constexpr (SEAMS) {  // the strip's edge columns, for the next stage's seam prologues
+            if (lane < 2 || lane >= kWave - 2) {
+                gpair* e = eout + f * (kEdgeRecord / 2);
+                e[0].a = out.d;
+                e[0].b = out.m;
+                e[1].a = out.n;
+                e[1].b = out.e;
+            }
+        }
 
         if constexpr (STAGE == 2) {  // wave speeds of U^{n+1} for the next dt (solver2d.hpp:67-83)
             const double rd = recip(out.d);
@@ -364,38 +476,40 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* 
 
 // ---------------------------------------------------------------------------------------------
 // launch table
-using StageFn = void (*)(const InstDesc*, Ctl*, const Task*, int, Partial*, int*);
+using StageFn = void (*)(const InstDesc*, Ctl*, const Task*, int, Partial*, int*, int, int);
 
-template <int S, int L, bool R>
+template <int S, int L, int M>
 constexpr StageFn stage_fn(int stage, bool corr) {
-    return stage == 1 ? (corr ? k_stage<S, L, 1, true, R> : k_stage<S, L, 1, false, R>)
-                      : (corr ? k_stage<S, L, 2, true, R> : k_stage<S, L, 2, false, R>);
+    return stage == 1 ? (corr ? k_stage<S, L, 1, true, M> : k_stage<S, L, 1, false, M>)
+                      : (corr ? k_stage<S, L, 2, true, M> : k_stage<S, L, 2, false, M>);
 }
 
-template <bool R>
+template <int M>
 StageFn pick_stage_t(int scheme, int lim, int stage, bool corr) {
     if (scheme == kRusanov) {
         switch (lim) {
-            case kLimNone: return stage_fn<kRusanov, kLimNone, R>(stage, corr);
-            case kLimUnlimited: return stage_fn<kRusanov, kLimUnlimited, R>(stage, corr);
-            case kLimMinmod: return stage_fn<kRusanov, kLimMinmod, R>(stage, corr);
-            default: return stage_fn<kRusanov, kLimVanLeer, R>(stage, corr);
+            case kLimNone: return stage_fn<kRusanov, kLimNone, M>(stage, corr);
+            case kLimUnlimited: return stage_fn<kRusanov, kLimUnlimited, M>(stage, corr);
+            case kLimMinmod: return stage_fn<kRusanov, kLimMinmod, M>(stage, corr);
+            default: return stage_fn<kRusanov, kLimVanLeer, M>(stage, corr);
         }
     }
     switch (lim) {
-        case kLimNone: return stage_fn<kHllc, kLimNone, R>(stage, corr);
-        case kLimUnlimited: return stage_fn<kHllc, kLimUnlimited, R>(stage, corr);
-        case kLimMinmod: return stage_fn<kHllc, kLimMinmod, R>(stage, corr);
-        default: return stage_fn<kHllc, kLimVanLeer, R>(stage, corr);
+        case kLimNone: return stage_fn<kHllc, kLimNone, M>(stage, corr);
+        case kLimUnlimited: return stage_fn<kHllc, kLimUnlimited, M>(stage, corr);
+        case kLimMinmod: return stage_fn<kHllc, kLimMinmod, M>(stage, corr);
+        default: return stage_fn<kHllc, kLimVanLeer, M>(stage, corr);
     }
 }
 
-// row_waves > 0: row-group kernels (W = row_waves wavefronts per workgroup); 0: strips
-StageFn pick_stage(int scheme, int lim, int stage, bool corr, int row_waves) {
-    return row_waves > 0 ? pick_stage_t<true>(scheme, lim, stage, corr)
-                         : pick_stage_t<false>(scheme, lim, stage, corr);
+// mode: StageMode (strips, row groups, seams)
+StageFn pick_stage(int scheme, int lim, int stage, bool corr, int mode) {
+    switch (mode) {
+        case kRowGroups: return pick_stage_t<kRowGroups>(scheme, lim, stage, corr);
+        case kSeams: return pick_stage_t<kSeams>(scheme, lim, stage, corr);
+        default: return pick_stage_t<kStrips>(scheme, lim, stage, corr);
+    }
 }
-
 
 }  // namespace NPDE_ARITH_NS
 }  // namespace npde

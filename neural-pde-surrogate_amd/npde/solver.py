@@ -322,6 +322,16 @@ class Batch:
             raise ValueError(f"arithmetic must be 'exact' or 'fast', not {mode!r}")
         check(self._h, self._lib.npde_set_arithmetic(self._h, self.ARITH[mode]))
 
+    TASK_LAYOUTS = ("strips", "row_groups", "seams")
+
+    @property
+    def task_layout(self) -> str:
+        """Task layout of the stage kernels under the current tuning: 'strips' (60 columns and a halo per
+        wavefront), 'row_groups' or 'seams' (64 columns per wavefront, which solves its strip boundaries first)."""
+        m = C.c_int()
+        check(self._h, self._lib.npde_get_task_layout(self._h, C.byref(m)))
+        return self.TASK_LAYOUTS[m.value]
+
     def autotune(self, cache_path: str | None = None) -> dict:
         t = Tuning()
         check(self._h, self._lib.npde_autotune(self._h, cache_path.encode() if cache_path else None, C.byref(t)))

@@ -317,13 +317,15 @@ def test_row_groups_match_oracle_and_strips(npde_mod, oracle, nx, nys):
         U = riemann(k + 5, max(nx, ny))[0][:, :ny, :nx]
         cases.append((oracle2d.Grid.make(nx, ny, y=(0.0, ny / nx), bc_x="periodic", bc_y=k % 2), np.ascontiguousarray(U)))
     results = {}
-    for rg, rows in ((2, 0), (2, 7), (1, 5), (0, 16)):
+    for rg, rows, seams in ((2, 0, 1), (2, 7, 1), (1, 5, 1), (0, 16, 0), (0, 11, 1)):
         with npde_mod.Batch([to_npde_grid(npde_mod, g) for g, _ in cases], "hllc", "vanleer", 0.5) as b:
-            b.set_tuning(row_groups=rg, rows_per_task=rows)
+            b.set_tuning(row_groups=rg, rows_per_task=rows, seams=seams)
+            expect = "row_groups" if rg == 2 or (rg == 1 and nx <= 256) else ("seams" if seams else "strips")
+            assert b.task_layout == expect
             for k, (_, U) in enumerate(cases):
                 b.set_state(k, U)
             b.advance_to(T)
-            results[(rg, rows)] = [(b.time(k), b.get_state(k)) for k in range(len(cases))]
+            results[(rg, rows, seams)] = [(b.time(k), b.get_state(k)) for k in range(len(cases))]
     for k, (g, U) in enumerate(cases):
         a = U.copy()
         ro = oracle.run(g, "hllc", "vanleer", 0.5, a, T)
@@ -331,6 +333,79 @@ def test_row_groups_match_oracle_and_strips(npde_mod, oracle, nx, nys):
             (t, steps), got = res[k]
             assert steps == ro["steps"] and t == ro["t"], (key, k)
             assert same(got, a), (key, k)
+
+
+@pytest.mark.parametrize("scheme,lim", [("hllc", "vanleer"), ("rusanov", "minmod")])
+def test_seams_match_oracle_and_strips(npde_mod, oracle, scheme, lim):
+    """Seam mode (widths that are multiples of 64: 64-column strips, each wavefront solving its strip
+    boundaries first) gives the oracle's bits, the same as 60-column strips, on one batch of mixed widths
+    (64-256), periodic and transmissive x and y, and ragged band heights."""
+    from npde.families import riemann
+    T = 0.01
+    shapes = [(64, 64, "transmissive", "periodic"), (128, 96, "periodic", "transmissive"),
+              (192, 70, "transmissive", "transmissive"), (256, 128, "periodic", "periodic")]
+    cases = []
+    for k, (nx, ny, bx, by) in enumerate(shapes):
+        U = riemann(k + 21, max(nx, ny))[0][:, :ny, :nx]
+        cases.append((oracle2d.Grid.make(nx, ny, y=(0.0, ny / nx), bc_x=bx, bc_y=by), np.ascontiguousarray(U)))
+    results = {}
+    for key in (dict(seams=1, rows_per_task=0), dict(seams=1, rows_per_task=9, waves_per_block=2),
+                dict(seams=1, rows_per_task=64, waves_per_block=1, use_graph=0), dict(seams=0, rows_per_task=16),
+                dict(seams=1, rows_per_task=128)):
+        with npde_mod.Batch([to_npde_grid(npde_mod, g) for g, _ in cases], scheme, lim, 0.5) as b:
+            b.set_tuning(row_groups=0, **key)
+            # seam bands keep their seam solves in LDS: at most 64 rows
+            assert b.task_layout == ("seams" if key["seams"] and key["rows_per_task"] <= 64 else "strips")
+            for k, (_, U) in enumerate(cases):
+                b.set_state(k, U)
+            b.advance_to(T)
+            results[str(key)] = [(b.time(k), b.get_state(k)) for k in range(len(cases))]
+    for k, (g, U) in enumerate(cases):
+        a = U.copy()
+        ro = oracle.run(g, scheme, lim, 0.5, a, T)
+        for key, res in results.items():
+            (t, steps), got = res[k]
+            assert steps == ro["steps"] and t == ro["t"], (key, k)
+            assert same(got, a), (key, k)
+
+
+def test_seams_step_a_state_written_in_place_between_calls(npde_mod, oracle):
+    """Seam mode keeps copies of the strips' edge columns between the stage kernels.  A state written in
+    place through the device pointer between two calls (which the library cannot see) must still be the
+    state the next call steps: the first seam pass of every call gathers from the state itself."""
+    import torch
+    from npde.families import riemann
+    n, nsteps = 128, 6
+    g = oracle2d.Grid.make(n, n, bc_x="transmissive", bc_y="periodic")
+    UA = np.ascontiguousarray(riemann(31, n)[0])
+    UB = np.ascontiguousarray(riemann(32, n)[0])
+    with npde_mod.Batch([to_npde_grid(npde_mod, g)], "hllc", "vanleer", 0.5) as b:
+        b.set_tuning(row_groups=0, seams=1, steps_per_graph=4)
+        assert b.task_layout == "seams"
+        b.set_state(0, UA)
+        b.advance_steps(nsteps, 0.5)
+        view = b.state_view()
+        view[0].copy_(torch.from_numpy(UB).to(view.device))
+        torch.cuda.synchronize()
+        b.advance_steps(nsteps, 0.5)
+        got = b.get_state(0)
+    a = UB.copy()
+    for _ in range(nsteps):
+        dt = oracle.compute_dt(g, 0.5, a)[0]
+        oracle.step(g, "hllc", "vanleer", a, dt)
+    assert same(got, a)
+
+
+def test_seam_layout_is_the_default_for_wide_instances(npde_mod):
+    """The default tuning runs 512-wide instances (the benchmark batch's width) in seam mode, keeps
+    row groups for periodic widths up to 256 and 60-column strips for widths that are not multiples of 64."""
+    def layout(nx, bc_x):
+        with npde_mod.Batch([npde_mod.make_grid(nx, 64, bc_x=bc_x, bc_y=1)], "hllc", "vanleer", 0.5) as b:
+            return b.task_layout
+    assert layout(512, 1) == "seams"
+    assert layout(256, 0) == "seams"
+    assert layout(256, 1) == "row_groups"
+    assert layout(100, 1) == "strips"
 
 
 def test_trajectory_invariants_of_check_trajectory(npde_mod):

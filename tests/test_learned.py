@@ -176,9 +176,13 @@ def test_network_is_translation_equivariant(cuda):
 
 
 @pytest.mark.gpu
-def test_corrected_hip_step_equals_torch_step(npde_mod, cuda):
+@pytest.mark.parametrize("n,tuning,layout", [(64, {}, "row_groups"), (128, dict(row_groups=0), "seams"),
+                                            (128, dict(row_groups=0, seams=0), "strips")])
+def test_corrected_hip_step_equals_torch_step(npde_mod, cuda, n, tuning, layout):
+    """The learned increments enter the HIP step exactly as in the differentiable torch step, in every
+    task layout (the seam pre-pass adds the increment of the faces it solves)."""
     from npde.learned import torch_solver as ts
-    n, B = 64, 3
+    B = 3
     U0 = torch.from_numpy(riemann_batch(n, range(B))).to(cuda)
     cx = (0.02 * torch.randn(B, 4, n, n, dtype=torch.float64, device=cuda)).contiguous()
     cy = (0.02 * torch.randn(B, 4, n, n, dtype=torch.float64, device=cuda)).contiguous()
@@ -188,6 +192,9 @@ def test_corrected_hip_step_equals_torch_step(npde_mod, cuda):
     plain = ts.step(U0, dt, 1 / n, 1 / n)
     torch.cuda.synchronize()
     with npde_mod.Batch(grids, "hllc", "vanleer", 0.5) as b:
+        if tuning:
+            b.set_tuning(**tuning)
+        assert b.task_layout == layout
         for k in range(B):
             b.set_state(k, U0[k].cpu().numpy())
             b.set_flux_correction(k, cx[k].data_ptr(), cy[k].data_ptr())
