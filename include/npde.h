@@ -67,8 +67,8 @@ typedef struct {
     int use_graph;        /* 0: plain stream launches */
     int row_groups;       /* when every instance is periodic in x with the same width 64*W, one workgroup of
                              W wavefronts per band exchanges edge columns through LDS instead of recomputing
-                             a halo: 1 (default) for W <= 4, 2 for W <= 8; 0: always 60-column strips with a
-                             2-column halo */
+                             a halo: 1 (default) for W = 1, 2 for W <= 8; 0: never (row groups take
+                             precedence over seams where both apply) */
     int seams;            /* when every instance's width is a multiple of 64, row groups do not apply and bands
                              are at most 64 rows, each wavefront owns a 64-column strip and first solves the
                              two strip boundaries of its band (from edge columns the previous stage kernel

@@ -123,13 +123,14 @@ int band_rows(const npde_solver* s, int strip_cols) {
 
 // Row groups apply when every instance is periodic in x with the same width 64*W: then one workgroup
 // of W wavefronts covers a band and no halo column is recomputed.  row_groups = 1 (the default) uses
-// them for W <= 4, where they measure faster than strips (64^2-256^2: +12-25%); at W = 8 a workgroup
-// of 8 wavefronts with 135-164 VGPRs is alone on its CU and waits at every barrier, and strips are
-// faster, so W <= 8 needs row_groups = 2.
+// them for W = 1 (64 columns), where they measure slightly faster than seams (18.1 against 17.8 G
+// cell-updates/s on 4096 x 64^2); for W = 2 .. 4 seams measure as fast or faster, and at W = 8 a
+// workgroup of 8 wavefronts with 135-164 VGPRs is alone on its CU and waits at every barrier.
+// row_groups = 2 allows W <= 8.
 int row_group_waves_for(const npde_solver* s, const npde_tuning& tune) {
     if (tune.row_groups <= 0 || s->n == 0) return 0;
     const int nx = s->grids[0].nx;
-    const int wmax = tune.row_groups >= 2 ? kMaxRowWaves : kMaxRowWaves / 2;
+    const int wmax = tune.row_groups >= 2 ? kMaxRowWaves : 1;
     if (nx % kWave != 0 || nx / kWave > wmax) return 0;
     for (const auto& g : s->grids)
         if (g.nx != nx || g.bc_x != NPDE_BC_PERIODIC) return 0;

@@ -17,8 +17,9 @@
 //     workgroup of W wavefronts covers a whole band of rows and every lane owns a column; the edge
 //     lanes' x neighbours come from the adjacent wavefronts of the workgroup (periodically) through
 //     a few LDS slots, read as the `old` operand of the DPP shift, so no halo column is recomputed;
-//     two workgroup barriers per row order the exchanges (the host uses it up to W = 4 by default:
-//     an 8-wavefront workgroup at these register counts is alone on its CU and waits at each barrier);
+//     two workgroup barriers per row order the exchanges (by default the host uses it for W = 1 only,
+//     where it beats the seam mode below; an 8-wavefront workgroup at these register counts is alone
+//     on its CU and waits at each barrier);
 //   * seam mode (SEAMS): for widths that are a multiple of 64, every lane owns a column of a 64-column
 //     strip.  Before its march, the wavefront solves the two strip boundaries ("seams") of its band
 //     in parallel (one lane per seam and row) into LDS: the primitives of the outside neighbour

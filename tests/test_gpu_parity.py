@@ -320,7 +320,7 @@ def test_row_groups_match_oracle_and_strips(npde_mod, oracle, nx, nys):
     for rg, rows, seams in ((2, 0, 1), (2, 7, 1), (1, 5, 1), (0, 16, 0), (0, 11, 1)):
         with npde_mod.Batch([to_npde_grid(npde_mod, g) for g, _ in cases], "hllc", "vanleer", 0.5) as b:
             b.set_tuning(row_groups=rg, rows_per_task=rows, seams=seams)
-            expect = "row_groups" if rg == 2 or (rg == 1 and nx <= 256) else ("seams" if seams else "strips")
+            expect = "row_groups" if rg == 2 or (rg == 1 and nx == 64) else ("seams" if seams else "strips")
             assert b.task_layout == expect
             for k, (_, U) in enumerate(cases):
                 b.set_state(k, U)
@@ -397,14 +397,17 @@ def test_seams_step_a_state_written_in_place_between_calls(npde_mod, oracle):
 
 
 def test_seam_layout_is_the_default_for_wide_instances(npde_mod):
-    """The default tuning runs 512-wide instances (the benchmark batch's width) in seam mode, keeps
-    row groups for periodic widths up to 256 and 60-column strips for widths that are not multiples of 64."""
+    """The default tuning runs widths that are multiples of 64 (the benchmark batch's 512 among them) in
+    seam mode, except 64-wide periodic instances (row groups of one wavefront), and other widths in
+    60-column strips."""
     def layout(nx, bc_x):
         with npde_mod.Batch([npde_mod.make_grid(nx, 64, bc_x=bc_x, bc_y=1)], "hllc", "vanleer", 0.5) as b:
             return b.task_layout
     assert layout(512, 1) == "seams"
     assert layout(256, 0) == "seams"
-    assert layout(256, 1) == "row_groups"
+    assert layout(256, 1) == "seams"
+    assert layout(64, 1) == "row_groups"
+    assert layout(64, 0) == "seams"
     assert layout(100, 1) == "strips"
 
 
