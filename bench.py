@@ -128,7 +128,7 @@ def main():
     ap.add_argument("--wpb", type=int, default=None, help="fix the waves per block (skips autotuning)")
     ap.add_argument("--row-groups", type=int, default=None, choices=(0, 1, 2),
                     help="fix the row-group mode (skips autotuning)")
-    ap.add_argument("--seams", type=int, default=None, choices=(0, 1),
+    ap.add_argument("--seams", type=int, default=None, choices=(0, 1, 2),
                     help="fix the seam mode: 64-column strips that solve their seams first (skips autotuning)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 1 GiB device copy (calibrates profiler byte counters)")

@@ -111,13 +111,15 @@ bool valid_grid(const npde_grid_desc& g, std::string& why) {
 }
 
 // Band height: the tuned value, or (rows_per_task <= 0) the tallest band in [4, 64] that still gives
-// about 4096 tasks, so that small grids are not bound by one wavefront marching many rows.
+// about 1024 tasks (one per SIMD), so that small grids are not bound by one wavefront marching many
+// rows; shorter bands recompute their 2 + 2 edge rows more often (a single 1024^2 instance: 13.0 G
+// cell-updates/s at 16 rows, 10.6 G at 4).
 int band_rows(const npde_solver* s, int strip_cols) {
     if (s->tune.rows_per_task > 0) return s->tune.rows_per_task;
     long long rows_total = 0;
     for (const auto& g : s->grids) rows_total += (long long)((g.nx + strip_cols - 1) / strip_cols) * g.ny;
     int r = 64;
-    while (r > 4 && rows_total / r < 4096) r /= 2;
+    while (r > 4 && rows_total / r < 1024) r /= 2;
     return r;
 }
 
@@ -141,13 +143,20 @@ int row_group_waves(const npde_solver* s) { return row_group_waves_for(s, s->tun
 // Seam mode applies when every instance's width is a multiple of 64, row groups do not apply and bands
 // are at most 64 rows (each wavefront keeps its band's seam solves in LDS): each wavefront then owns a
 // 64-column strip and solves the two strip boundaries of its band itself before its march, so no halo
-// column is recomputed and no wavefront waits for another.
+// column is recomputed and no wavefront waits for another.  seams = 1 (the default) uses it for
+// batches of at least kSeamMinCells cells: the seam solves lengthen every wavefront's serial path,
+// which a batch too small to fill the GPU pays in full (a single 256^2 instance: 38 against 28 us per
+// step in strips); seams = 2 uses it at any size.
+constexpr long long kSeamMinCells = 1 << 20;
 bool seams_apply_for(const npde_solver* s, const npde_tuning& tune) {
     if (tune.seams <= 0 || s->n == 0 || row_group_waves_for(s, tune) > 0) return false;
     if (tune.rows_per_task > 64) return false;
-    for (const auto& g : s->grids)
+    long long cells = 0;
+    for (const auto& g : s->grids) {
         if (g.nx % kWave != 0) return false;
-    return true;
+        cells += (long long)g.nx * g.ny;
+    }
+    return tune.seams >= 2 || cells >= kSeamMinCells;
 }
 
 // Task list and partial slots.  Strips and seams: one task and one partial per wavefront.  Row groups:
@@ -1028,8 +1037,8 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
     std::vector<std::pair<int, int>> layouts;
     if (row_group_waves_for(s, probe) > 0) layouts.push_back({2, 0});
     probe.row_groups = 0;
-    probe.seams = 1;
-    if (seams_apply_for(s, probe)) layouts.push_back({0, 1});
+    probe.seams = 2;
+    if (seams_apply_for(s, probe)) layouts.push_back({0, 2});
     layouts.push_back({0, 0});
     for (const auto& lay : layouts)
     for (int r : rows_c) {

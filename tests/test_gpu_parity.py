@@ -317,7 +317,7 @@ def test_row_groups_match_oracle_and_strips(npde_mod, oracle, nx, nys):
         U = riemann(k + 5, max(nx, ny))[0][:, :ny, :nx]
         cases.append((oracle2d.Grid.make(nx, ny, y=(0.0, ny / nx), bc_x="periodic", bc_y=k % 2), np.ascontiguousarray(U)))
     results = {}
-    for rg, rows, seams in ((2, 0, 1), (2, 7, 1), (1, 5, 1), (0, 16, 0), (0, 11, 1)):
+    for rg, rows, seams in ((2, 0, 2), (2, 7, 2), (1, 5, 2), (0, 16, 0), (0, 11, 2)):
         with npde_mod.Batch([to_npde_grid(npde_mod, g) for g, _ in cases], "hllc", "vanleer", 0.5) as b:
             b.set_tuning(row_groups=rg, rows_per_task=rows, seams=seams)
             expect = "row_groups" if rg == 2 or (rg == 1 and nx == 64) else ("seams" if seams else "strips")
@@ -349,9 +349,9 @@ def test_seams_match_oracle_and_strips(npde_mod, oracle, scheme, lim):
         U = riemann(k + 21, max(nx, ny))[0][:, :ny, :nx]
         cases.append((oracle2d.Grid.make(nx, ny, y=(0.0, ny / nx), bc_x=bx, bc_y=by), np.ascontiguousarray(U)))
     results = {}
-    for key in (dict(seams=1, rows_per_task=0), dict(seams=1, rows_per_task=9, waves_per_block=2),
-                dict(seams=1, rows_per_task=64, waves_per_block=1, use_graph=0), dict(seams=0, rows_per_task=16),
-                dict(seams=1, rows_per_task=128)):
+    for key in (dict(seams=2, rows_per_task=0), dict(seams=2, rows_per_task=9, waves_per_block=2),
+                dict(seams=2, rows_per_task=64, waves_per_block=1, use_graph=0), dict(seams=0, rows_per_task=16),
+                dict(seams=2, rows_per_task=128)):
         with npde_mod.Batch([to_npde_grid(npde_mod, g) for g, _ in cases], scheme, lim, 0.5) as b:
             b.set_tuning(row_groups=0, **key)
             # seam bands keep their seam solves in LDS: at most 64 rows
@@ -380,7 +380,7 @@ def test_seams_step_a_state_written_in_place_between_calls(npde_mod, oracle):
     UA = np.ascontiguousarray(riemann(31, n)[0])
     UB = np.ascontiguousarray(riemann(32, n)[0])
     with npde_mod.Batch([to_npde_grid(npde_mod, g)], "hllc", "vanleer", 0.5) as b:
-        b.set_tuning(row_groups=0, seams=1, steps_per_graph=4)
+        b.set_tuning(row_groups=0, seams=2, steps_per_graph=4)
         assert b.task_layout == "seams"
         b.set_state(0, UA)
         b.advance_steps(nsteps, 0.5)
@@ -397,18 +397,23 @@ def test_seams_step_a_state_written_in_place_between_calls(npde_mod, oracle):
 
 
 def test_seam_layout_is_the_default_for_wide_instances(npde_mod):
-    """The default tuning runs widths that are multiples of 64 (the benchmark batch's 512 among them) in
-    seam mode, except 64-wide periodic instances (row groups of one wavefront), and other widths in
-    60-column strips."""
-    def layout(nx, bc_x):
-        with npde_mod.Batch([npde_mod.make_grid(nx, 64, bc_x=bc_x, bc_y=1)], "hllc", "vanleer", 0.5) as b:
+    """The default tuning runs batches of at least 2^20 cells whose widths are multiples of 64 (the
+    benchmark batch among them) in seam mode, except 64-wide periodic instances (row groups of one
+    wavefront); smaller batches and other widths run in 60-column strips."""
+    def layout(nx, bc_x, count, **tuning):
+        grids = [npde_mod.make_grid(nx, 64, bc_x=bc_x, bc_y=1)] * count
+        with npde_mod.Batch(grids, "hllc", "vanleer", 0.5) as b:
+            if tuning:
+                b.set_tuning(**tuning)
             return b.task_layout
-    assert layout(512, 1) == "seams"
-    assert layout(256, 0) == "seams"
-    assert layout(256, 1) == "seams"
-    assert layout(64, 1) == "row_groups"
-    assert layout(64, 0) == "seams"
-    assert layout(100, 1) == "strips"
+    assert layout(512, 1, 32) == "seams"
+    assert layout(256, 0, 64) == "seams"
+    assert layout(256, 1, 64) == "seams"
+    assert layout(64, 1, 256) == "row_groups"
+    assert layout(64, 0, 256) == "seams"
+    assert layout(100, 1, 256) == "strips"
+    assert layout(256, 0, 1) == "strips"              # 2^14 cells: too small to fill the GPU
+    assert layout(256, 0, 1, seams=2) == "seams"
 
 
 def test_trajectory_invariants_of_check_trajectory(npde_mod):
