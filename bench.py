@@ -26,6 +26,7 @@ sys.path.insert(0, str(ROOT / "oracle"))
 
 N_INST, N = 64, 512
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 BYTES_STAGE1, BYTES_STAGE2 = 64, 96  # algorithmic HBM bytes per cell per stage (binary64, 4 variables)
 
 
@@ -60,18 +61,36 @@ def cpu_baseline(U0, seconds):
                       f"from t = 0 in {elapsed:.2f} s on {threads} OpenMP threads"}
 
 
-def traffic(layout):
-    """HBM bytes per stage-2 launch in this task layout from the committed rocprofv3 counter summary
-    (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a plain device copy of known size gives), or None
-    without a measurement of that layout."""
+def counter(name, layout):
+    """A per-launch stage-2 figure for this task layout from the committed rocprofv3 counter summary
+    (lines `stage2_<name> (<layout>) = value`), or None without a measurement of that layout.  The
+    HBM bytes are FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a plain device copy of known size gives."""
     f = ROOT / "profiles" / "stage_kernels_64x512_counters.md"
     if not f.exists():
         return None
-    key = f"stage2_hbm_bytes_per_launch ({layout}) = "
+    key = f"stage2_{name} ({layout}) = "
     for line in f.read_text().splitlines():
         if line.startswith(key):
             return float(line[len(key):])
     return None
+
+
+def valu_roofline(layout, launch_ms):
+    """Stage 2 against the FP64 VALU: the profiled instruction counts per launch over the live launch
+    time.  Issue peak: 256 CUs x 4 SIMDs, one wave64 VALU instruction per SIMD per 4 clocks at 2.4 GHz;
+    the counts weigh the quarter-rate FP64 transcendentals 4x (SQ_ACTIVE_INST_VALU)."""
+    active = counter("valu_active_per_launch", layout)
+    if active is None:
+        return None
+    peak = 256 * 4 * 2.4e9 / 4
+    rate = active / (launch_ms * 1e-3)
+    out = {"bound": "fp64 valu issue", "achieved": rate, "peak": peak, "unit": "wave-instr/s", "frac": rate / peak,
+           "insts_per_launch": counter("valu_insts_per_launch", layout), "active_per_launch": active}
+    flops = counter("fp64_flops_per_launch", layout)
+    if flops is not None:
+        out["fp64_tflops"] = flops / (launch_ms * 1e-3) / 1e12
+        out["fp64_peak_tflops"] = FP64_PEAK_TFLOPS
+    return out
 
 
 def fast_variant(b, ics, steps, warmup):
@@ -200,8 +219,9 @@ def main():
                    "instances": N_INST, "n": N, "tuning": tuning},
         "roofline": {"bound": "hbm", "kernel": f"npde::exact::k_stage<HLLC, vanLeer, 2, false, {layout}>",
                      "achieved": ach2, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach2 / HBM_PEAK_GBS,
-                     "traffic": traffic(layout), "bytes_per_cell": BYTES_STAGE2, "launch_ms": ms2,
+                     "traffic": counter("hbm_bytes_per_launch", layout), "bytes_per_cell": BYTES_STAGE2, "launch_ms": ms2,
                      "stage1_launch_ms": ms1, "stage1_achieved": ach1},
+        "valu_roofline": valu_roofline(layout, ms2),
         "cpu_baseline": cpu,
         "fast_arith_variant": fast,
     }
