@@ -2,8 +2,10 @@
 // (euler_kernels.hip) and the fast-arithmetic build (euler_kernels_fast.hip); the arithmetic comes
 // from euler_numerics.hpp, whose namespace (npde::exact or npde::fast) this file's kernels join.
 //
-// Stage kernel design (one wavefront = one task = 60 interior columns x a band of rows):
-//   * each lane owns one column; lanes 0,1 and 62,63 are the 2-column halo of the strip;
+// Stage kernel design (one wavefront = a strip of columns x a band of rows; the strip layout is
+// described first, the row-group and seam layouts below):
+//   * each lane owns one column; in the strip layout lanes 0,1 and 62,63 are the 2-column halo of a
+//     60-column strip;
 //   * the wavefront marches up its band one row per iteration, keeping a 3-row window of primitives
 //     in registers, so every cell's primitives, limited slopes and four face-side states
 //     (W +- dW/2 -> conserved -> primitive + sound speed) are computed exactly once per stage;
