@@ -172,6 +172,27 @@ def test_sod_transpose_is_exact(npde_mod):
     assert (X[2] == 0.0).all()
 
 
+def test_sod_transpose_is_exact_across_layouts(npde_mod):
+    """sod_x at 256 x 4 runs in seam mode (width a multiple of 64), sod_y at 4 x 256 in strips: the
+    transposed results are still exactly equal (test_solver2d.cpp:86-95's bar, across layouts)."""
+    gx, t_end, Ux = npde_mod.builtin_problem("sod_x", 256)
+    gy, _, Uy = npde_mod.builtin_problem("sod_y", 256)
+    with npde_mod.Batch([gx, gy], "hllc", "vanleer", 0.5) as b:
+        b.set_tuning(seams=2)
+        assert b.task_layout == "strips"   # one batch runs one layout: sod_y's width 4 rules seams out
+    X, Y = {}, {}
+    for g, U, out in ((gx, Ux, X), (gy, Uy, Y)):
+        with npde_mod.Batch([g], "hllc", "vanleer", 0.5) as b:
+            b.set_tuning(seams=2)
+            out["layout"] = b.task_layout
+            b.set_state(0, U)
+            b.advance_to(t_end)
+            out["steps"], out["U"] = b.time(0)[1], b.get_state(0)
+    assert (X["layout"], Y["layout"]) == ("seams", "strips")
+    assert X["steps"] == Y["steps"]
+    assert np.abs(X["U"] - np.transpose(Y["U"], (0, 2, 1))[[0, 2, 1, 3]]).max() == 0.0
+
+
 def test_lax_liu_256_published_step_count(npde_mod):
     g, t_end, U0 = npde_mod.builtin_problem("lax_liu3", 256)
     with npde_mod.Batch([g], "hllc", "vanleer", 0.5) as b:
@@ -394,6 +415,28 @@ def test_seams_step_a_state_written_in_place_between_calls(npde_mod, oracle):
         dt = oracle.compute_dt(g, 0.5, a)[0]
         oracle.step(g, "hllc", "vanleer", a, dt)
     assert same(got, a)
+
+
+def test_seams_host_driven_steps_match_oracle(npde_mod, oracle):
+    """Solver.compute_dt / Solver.step, one host call per step, in seam mode (every call's stage 1
+    gathers its seam columns from the state, stage 2 reads the edge arrays): the oracle's dt sequence
+    and bits, on a transmissive-x, periodic-y grid with Rusanov + minmod."""
+    from npde.families import riemann
+    n, ny, nsteps = 192, 80, 5
+    g = oracle2d.Grid.make(n, ny, y=(0.0, ny / n), bc_x="transmissive", bc_y="periodic")
+    U0 = np.ascontiguousarray(riemann(41, n)[0][:, :ny, :])
+    a = U0.copy()
+    with npde_mod.Batch([to_npde_grid(npde_mod, g)], "rusanov", "minmod", 0.5) as b:
+        b.set_tuning(seams=2, rows_per_task=16)
+        assert b.task_layout == "seams"
+        b.set_state(0, U0)
+        for _ in range(nsteps):
+            dt = float(b.compute_dts()[0])
+            dto = oracle.compute_dt(g, 0.5, a)[0]
+            assert dt == dto
+            b.step_all([dt])
+            oracle.step(g, "rusanov", "minmod", a, dto)
+        assert same(b.get_state(0), a)
 
 
 def test_seam_layout_is_the_default_for_wide_instances(npde_mod):
