@@ -70,7 +70,7 @@ struct npde_solver {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     npde_tuning tune{0, 4, 32, 1, 1, 1};
-    int mode = kStrips;  // StageMode of the current task list
+    int layout = kStrips;  // StageMode of the current task list
     int row_waves = 0;   // row groups: wavefronts per band
     int band = 0;        // band height of the current task list
     hipGraphExec_t graph = nullptr;
@@ -165,9 +165,9 @@ bool seams_apply_for(const npde_solver* s, const npde_tuning& tune) {
 void build_tasks(npde_solver* s) {
     s->htasks.clear();
     s->row_waves = row_group_waves(s);
-    s->mode = s->row_waves > 0 ? kRowGroups : (seams_apply_for(s, s->tune) ? kSeams : kStrips);
+    s->layout = s->row_waves > 0 ? kRowGroups : (seams_apply_for(s, s->tune) ? kSeams : kStrips);
     const int W = s->row_waves;
-    const int cols = s->mode == kStrips ? kStripCols : kWave;
+    const int cols = s->layout == kStrips ? kStripCols : kWave;
     const int rows = band_rows(s, cols);
     s->band = rows;
     for (int k = 0; k < s->n; ++k) {
@@ -199,7 +199,7 @@ npde_status upload_tasks(npde_solver* s) {
     // (nx/64) x ny x kEdgeRecord doubles
     auto edge_doubles = [](const InstDesc& D) { return size_t(2 * kEdgeRecord) * (D.nx / kWave) * D.ny; };
     size_t edge_need = 0;
-    if (s->mode == kSeams)
+    if (s->layout == kSeams)
         for (const auto& D : s->hdesc) edge_need += 2 * edge_doubles(D);
     if (edge_need > s->edge_cap) {
         if (s->dedge) (void)hipFree(s->dedge);
@@ -211,7 +211,7 @@ npde_status upload_tasks(npde_solver* s) {
     size_t off = 0;
     for (auto& D : s->hdesc) {
         D.edge[0] = D.edge[1] = nullptr;
-        if (s->mode != kSeams) continue;
+        if (s->layout != kSeams) continue;
         D.edge[0] = s->dedge + off;
         D.edge[1] = D.edge[0] + edge_doubles(D);
         off += 2 * edge_doubles(D);
@@ -268,7 +268,7 @@ npde_status fill_bad(npde_solver* s, npde_bad_cell* bad) {
 // graph, an iteration loop), whose input the caller may have changed since a stage 2 last wrote them.
 npde_status launch_stage_kernel(npde_solver* s, int stage, bool first_step) {
     const int compact = (stage == 2 || !first_step) ? 1 : 0;
-    HIPCHK(s, launch_stage(s->flux, s->lim, stage, s->has_corr, s->fast, s->tune.waves_per_block, s->mode,
+    HIPCHK(s, launch_stage(s->flux, s->lim, stage, s->has_corr, s->fast, s->tune.waves_per_block, s->layout,
                            s->row_waves, compact, s->band, s->ddesc, s->dctl, s->dtasks, int(s->htasks.size()),
                            s->dpart, s->dndone, s->stream));
     return NPDE_OK;
@@ -286,7 +286,7 @@ npde_status launch_step_pair(npde_solver* s, bool first_step) {
 }
 
 npde_status launch_wavespeed_and_finish(npde_solver* s) {
-    HIPCHK(s, launch_wavespeed(s->tune.waves_per_block, s->mode, s->row_waves, s->ddesc, s->dctl, s->dtasks,
+    HIPCHK(s, launch_wavespeed(s->tune.waves_per_block, s->layout, s->row_waves, s->ddesc, s->dctl, s->dtasks,
                                int(s->htasks.size()), s->dpart, s->dndone, s->stream));
     HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterWavespeed, s->dndone, s->stream));
     return NPDE_OK;
@@ -876,7 +876,7 @@ npde_status npde_get_task_layout(const npde_solver* s, int* layout) {
     static_assert(int(NPDE_TASKS_STRIPS) == kStrips && int(NPDE_TASKS_ROW_GROUPS) == kRowGroups &&
                       int(NPDE_TASKS_SEAMS) == kSeams,
                   "npde_task_layout mirrors StageMode");
-    *layout = s->mode;
+    *layout = s->layout;
     return NPDE_OK;
 }
 
