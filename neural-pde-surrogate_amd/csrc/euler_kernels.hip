@@ -7,6 +7,7 @@
 //  k_frame                             block-averaged float32 frame of U (trajectory output)
 //  k_finish                            per-instance reduction of the partials and run-state decision
 //  k_prep_host_step                    host-supplied dt for the next step
+//  k_begin_run                         run state of every instance at the start of a device-resident run
 //
 // All arithmetic is binary64 with the reference's operation order (euler_numerics.hpp); the file is
 // compiled with -ffp-contract=off.
@@ -249,6 +250,28 @@ __global__ void k_prep_host_step(Ctl* __restrict__ ctl, int n, const double* __r
     ctl[k].active = dt >= 0.0 ? 1 : 0;
 }
 
+// Start of a device-resident run (npde_advance_to / npde_run_trajectory): every instance's run state is
+// set on the device, so the host needs no copy of it before the run.  max_new_steps < 0: no step budget.
+// Thread 0 also clears the finished-instance counter.
+__global__ void k_begin_run(Ctl* __restrict__ ctl, int n, double t_start, double t_end, int nout, int record,
+                            long long max_new_steps, int* __restrict__ ndone) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) *ndone = 0;
+    if (k >= n) return;
+    Ctl& c = ctl[k];
+    c.step_limit = max_new_steps < 0 ? (long long)(~0ull >> 1) : c.steps + max_new_steps;
+    c.t_start = t_start;
+    c.t_end = t_end;
+    c.nout = nout;
+    c.k = 0;
+    c.mode = kModeAuto;
+    c.record = record;
+    c.pending = -1;
+    c.done = 0;
+    c.error = 0;
+    c.active = 0;
+}
+
 // Arithmetic self-test, element-wise: the shared-reciprocal quotient and the plain IEEE quotient,
 // and the square root as the kernels evaluate it.
 __global__ void k_selftest_arith(const double* __restrict__ x, const double* __restrict__ y, long long n,
@@ -334,6 +357,13 @@ hipError_t launch_finish(const InstDesc* desc, Ctl* ctl, const Partial* part, in
 hipError_t launch_block_average(const InstDesc* desc, int inst, int m, int nyc, float* out, hipStream_t s) {
     const int cells = m * nyc;
     hipLaunchKernelGGL(k_block_average, dim3((cells + 255) / 256), dim3(256), 0, s, desc, inst, m, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_begin_run(Ctl* ctl, int n, double t_start, double t_end, int nout, int record, long long max_new_steps,
+                            int* ndone, hipStream_t s) {
+    hipLaunchKernelGGL(k_begin_run, dim3((n + 255) / 256), dim3(256), 0, s, ctl, n, t_start, t_end, nout, record,
+                       max_new_steps, ndone);
     return hipGetLastError();
 }
 

@@ -34,6 +34,8 @@ hipError_t launch_finish(const InstDesc* desc, Ctl* ctl, const Partial* part, in
                          hipStream_t s);
 enum : int { kAfterStep = 0, kAfterWavespeed = 1, kAfterFrame = 2 };
 hipError_t launch_prep_host_step(Ctl* ctl, int n, const double* dts, hipStream_t s);
+hipError_t launch_begin_run(Ctl* ctl, int n, double t_start, double t_end, int nout, int record, long long max_new_steps,
+                            int* ndone, hipStream_t s);
 hipError_t launch_selftest_arith(const double* x, const double* y, long long n, double* qf, double* qi, double* rt,
                                  hipStream_t s);
 }  // namespace npde
@@ -338,27 +340,14 @@ npde_status ensure_graph(npde_solver* s, int nft) {
 // Device-resident loop shared by npde_advance_to and npde_run_trajectory.
 npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, bool record, npde_bad_cell* bad,
                      long long max_new_steps = -1) {
-    npde_status st = pull_ctl(s);
-    if (st != NPDE_OK) return st;
-    for (auto& c : s->hctl) {
-        c.step_limit = max_new_steps < 0 ? (long long)(~0ull >> 1) : c.steps + max_new_steps;
-        c.t_start = t_start;
-        c.t_end = t_end;
-        c.nout = nout;
-        c.k = 0;
-        c.mode = kModeAuto;
-        c.record = record ? 1 : 0;
-        c.pending = -1;
-        c.done = 0;
-        c.error = 0;
-        c.active = 0;
-    }
-    if ((st = push_ctl(s)) != NPDE_OK) return st;
-    HIPCHK(s, hipMemsetAsync(s->dndone, 0, sizeof(int), s->stream));
+    npde_status st = NPDE_OK;
     const int nft = record ? int(s->hftasks.size()) : 0;
     if (s->tune.use_graph) {
         if ((st = ensure_graph(s, nft)) != NPDE_OK) return st;
     }
+    // the run state is set on the device and read back once at the end: no host round trip before the run
+    HIPCHK(s, launch_begin_run(s->dctl, s->n, t_start, t_end, nout, record ? 1 : 0, max_new_steps, s->dndone,
+                               s->stream));
     HIPCHK(s, hipEventRecord(s->ev0, s->stream));
     if ((st = launch_wavespeed_and_finish(s)) != NPDE_OK) return st;
     int done = 0;
@@ -378,11 +367,11 @@ npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, boo
         if (done >= s->n) break;
     }
     HIPCHK(s, hipEventRecord(s->ev1, s->stream));
-    HIPCHK(s, hipEventSynchronize(s->ev1));
+    HIPCHK(s, hipMemcpyAsync(s->hctl.data(), s->dctl, sizeof(Ctl) * s->n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
     float ms = 0.f;
     HIPCHK(s, hipEventElapsedTime(&ms, s->ev0, s->ev1));
     s->last_ms = ms;
-    if ((st = pull_ctl(s)) != NPDE_OK) return st;
     return fill_bad(s, bad);
 }
 
