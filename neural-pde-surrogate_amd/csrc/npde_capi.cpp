@@ -75,6 +75,10 @@ struct npde_solver {
     int layout = kStrips;  // StageMode of the current task list
     int row_waves = 0;   // row groups: wavefronts per band
     int band = 0;        // band height of the current task list
+    // instances whose tasks are in the current task list: all of them, except during the tail of a
+    // device-resident run, when the list is rebuilt for the instances still stepping (retile_tail)
+    std::vector<char> listed;
+    bool subset = false;
     hipGraphExec_t graph = nullptr;
     int graph_key_frames = -1;  // nftasks the cached graph was captured with (0: no frame kernel)
     int graph_key_steps = 0;
@@ -113,15 +117,16 @@ bool valid_grid(const npde_grid_desc& g, std::string& why) {
 }
 
 // Band height: the tuned value, or (rows_per_task <= 0) the tallest band in [4, 64] that still gives
-// about 1024 tasks (one per SIMD), so that small grids are not bound by one wavefront marching many
-// rows; shorter bands recompute their 2 + 2 edge rows more often (a single 1024^2 instance: 13.0 G
-// cell-updates/s at 16 rows, 10.6 G at 4).
+// about 2048 tasks over the listed instances, so that small grids are not bound by one wavefront
+// marching many rows, while bands stay tall enough not to recompute their 2 + 2 edge rows too often
+// (measured best or within 5% of it from a single 512^2 instance to 64 x 512^2).
 int band_rows(const npde_solver* s, int strip_cols) {
     if (s->tune.rows_per_task > 0) return s->tune.rows_per_task;
     long long rows_total = 0;
-    for (const auto& g : s->grids) rows_total += (long long)((g.nx + strip_cols - 1) / strip_cols) * g.ny;
+    for (int k = 0; k < s->n; ++k)
+        if (s->listed[k]) rows_total += (long long)((s->grids[k].nx + strip_cols - 1) / strip_cols) * s->grids[k].ny;
     int r = 64;
-    while (r > 4 && rows_total / r < 1024) r /= 2;
+    while (r > 4 && rows_total / r < 2048) r /= 2;
     return r;
 }
 
@@ -133,11 +138,15 @@ int band_rows(const npde_solver* s, int strip_cols) {
 // row_groups = 2 allows W <= 8.
 int row_group_waves_for(const npde_solver* s, const npde_tuning& tune) {
     if (tune.row_groups <= 0 || s->n == 0) return 0;
-    const int nx = s->grids[0].nx;
-    const int wmax = tune.row_groups >= 2 ? kMaxRowWaves : 1;
-    if (nx % kWave != 0 || nx / kWave > wmax) return 0;
-    for (const auto& g : s->grids)
+    int nx = -1;
+    for (int k = 0; k < s->n; ++k) {
+        if (!s->listed[k]) continue;
+        const auto& g = s->grids[k];
+        if (nx < 0) nx = g.nx;
         if (g.nx != nx || g.bc_x != NPDE_BC_PERIODIC) return 0;
+    }
+    const int wmax = tune.row_groups >= 2 ? kMaxRowWaves : 1;
+    if (nx <= 0 || nx % kWave != 0 || nx / kWave > wmax) return 0;
     return nx / kWave;
 }
 int row_group_waves(const npde_solver* s) { return row_group_waves_for(s, s->tune); }
@@ -154,7 +163,9 @@ bool seams_apply_for(const npde_solver* s, const npde_tuning& tune) {
     if (tune.seams <= 0 || s->n == 0 || row_group_waves_for(s, tune) > 0) return false;
     if (tune.rows_per_task > 64) return false;
     long long cells = 0;
-    for (const auto& g : s->grids) {
+    for (int k = 0; k < s->n; ++k) {
+        if (!s->listed[k]) continue;
+        const auto& g = s->grids[k];
         if (g.nx % kWave != 0) return false;
         cells += (long long)g.nx * g.ny;
     }
@@ -175,6 +186,11 @@ void build_tasks(npde_solver* s) {
     for (int k = 0; k < s->n; ++k) {
         InstDesc& D = s->hdesc[k];
         const int first = int(s->htasks.size());
+        if (!s->listed[k]) {  // not stepping in this list: no tasks, no partials
+            D.task0 = W > 0 ? first * W : first;
+            D.ntasks = 0;
+            continue;
+        }
         if (W > 0) {
             for (int j0 = 0; j0 < D.ny; j0 += rows) s->htasks.push_back({k, 0, j0, std::min(j0 + rows, D.ny)});
             D.task0 = first * W;
@@ -240,6 +256,18 @@ void drop_graph(npde_solver* s) {
     if (s->graph) (void)hipGraphExecDestroy(s->graph);
     s->graph = nullptr;
     s->graph_key_frames = -1;
+}
+
+// The task list for the instances in mask (band height and layout chosen for them), and back to all.
+npde_status list_instances(npde_solver* s, const std::vector<char>& mask) {
+    s->listed = mask;
+    s->subset = std::find(mask.begin(), mask.end(), 0) != mask.end();
+    drop_graph(s);
+    return upload_tasks(s);
+}
+npde_status list_all(npde_solver* s) {
+    if (!s->subset) return NPDE_OK;
+    return list_instances(s, std::vector<char>(s->n, 1));
 }
 
 // to_primitive on the host (same operations as the device), for error messages
@@ -337,10 +365,39 @@ npde_status ensure_graph(npde_solver* s, int nft) {
     return NPDE_OK;
 }
 
+// Tail of a device-resident run: most instances have finished, and the bands chosen for the whole batch
+// leave the rest latency-bound (4 x 256^2 instances step in 36 us with 4-row bands, 232 us with 64-row
+// bands).  Rebuild the task list -- band height and layout chosen again -- for the instances still
+// stepping, when their remaining work is worth a new graph: more than two graph launches at their current
+// dt.  listed_n becomes the number still stepping either way, so the next look comes when half of them
+// have finished.  Every layout gives the same bits.
+npde_status retile_tail(npde_solver* s, int nft, int& listed_n) {
+    HIPCHK(s, hipMemcpy(s->hctl.data(), s->dctl, sizeof(Ctl) * s->n, hipMemcpyDeviceToHost));
+    std::vector<char> mask(s->n, 0);
+    int active = 0;
+    double remaining = 0.0;
+    for (int k = 0; k < s->n; ++k) {
+        const Ctl& c = s->hctl[k];
+        if (c.done) continue;
+        mask[k] = 1;
+        ++active;
+        double steps = c.dt > 0.0 ? (c.t_end - c.t) / c.dt : 0.0;
+        steps = std::min(steps, double(c.step_limit - c.steps));
+        remaining = std::max(remaining, steps);
+    }
+    listed_n = active;
+    const int K = std::max(1, s->tune.steps_per_graph);
+    if (active == 0 || remaining < 2.0 * K) return NPDE_OK;
+    npde_status st = list_instances(s, mask);
+    if (st == NPDE_OK && s->tune.use_graph) st = ensure_graph(s, nft);
+    return st;
+}
+
 // Device-resident loop shared by npde_advance_to and npde_run_trajectory.
 npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, bool record, npde_bad_cell* bad,
                      long long max_new_steps = -1) {
-    npde_status st = NPDE_OK;
+    npde_status st = list_all(s);
+    if (st != NPDE_OK) return st;
     const int nft = record ? int(s->hftasks.size()) : 0;
     if (s->tune.use_graph) {
         if ((st = ensure_graph(s, nft)) != NPDE_OK) return st;
@@ -351,6 +408,7 @@ npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, boo
     HIPCHK(s, hipEventRecord(s->ev0, s->stream));
     if ((st = launch_wavespeed_and_finish(s)) != NPDE_OK) return st;
     int done = 0;
+    int listed_n = s->n;  // instances in the current task list
     const int K = std::max(1, s->tune.steps_per_graph);
     // A state whose wave speed is infinite gives dt = 0 and would never reach its target (the reference
     // loops forever there too); bound the loop instead.
@@ -365,6 +423,8 @@ npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, boo
         HIPCHK(s, hipMemcpyAsync(&done, s->dndone, sizeof(int), hipMemcpyDeviceToHost, s->stream));
         HIPCHK(s, hipStreamSynchronize(s->stream));
         if (done >= s->n) break;
+        if (2 * (s->n - done) <= listed_n && listed_n > 1 && (st = retile_tail(s, nft, listed_n)) != NPDE_OK)
+            return st;
     }
     HIPCHK(s, hipEventRecord(s->ev1, s->stream));
     HIPCHK(s, hipMemcpyAsync(s->hctl.data(), s->dctl, sizeof(Ctl) * s->n, hipMemcpyDeviceToHost, s->stream));
@@ -372,6 +432,7 @@ npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, boo
     float ms = 0.f;
     HIPCHK(s, hipEventElapsedTime(&ms, s->ev0, s->ev1));
     s->last_ms = ms;
+    if ((st = list_all(s)) != NPDE_OK) return st;
     return fill_bad(s, bad);
 }
 
@@ -430,6 +491,7 @@ npde_status npde_batch_create(int n, const npde_grid_desc* grids, int flux, int 
     s->cfl = cfl;
     s->grids.assign(grids, grids + n);
     s->hdesc.resize(n);
+    s->listed.assign(n, 1);
     s->hctl.assign(n, Ctl{});
     s->state_off.resize(n);
     size_t total = 0;
@@ -620,7 +682,8 @@ npde_status npde_state_pointer(npde_solver* s, int inst, double** U, long long* 
 
 npde_status npde_compute_dt(npde_solver* s, double* dt, npde_bad_cell* bad) {
     if (!s || !dt) return fail(s, NPDE_ERR_INVALID, "null argument");
-    npde_status st = set_host_mode(s);
+    npde_status st = list_all(s);
+    if (st == NPDE_OK) st = set_host_mode(s);
     if (st != NPDE_OK) return st;
     if ((st = launch_wavespeed_and_finish(s)) != NPDE_OK) return st;
     if ((st = pull_ctl(s)) != NPDE_OK) return st;
@@ -636,7 +699,8 @@ npde_status npde_compute_dt(npde_solver* s, double* dt, npde_bad_cell* bad) {
 // of such steps (the caller may have changed the state before it).
 static npde_status host_step(npde_solver* s, const double* dt, bool first_step) {
     if (!s || !dt) return fail(s, NPDE_ERR_INVALID, "null argument");
-    npde_status st = set_host_mode(s);
+    npde_status st = list_all(s);
+    if (st == NPDE_OK) st = set_host_mode(s);
     if (st != NPDE_OK) return st;
     HIPCHK(s, hipMemcpyAsync(s->ddts, dt, sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
     HIPCHK(s, launch_prep_host_step(s->dctl, s->n, s->ddts, s->stream));
@@ -842,6 +906,7 @@ npde_status npde_set_tuning(npde_solver* s, const npde_tuning* t) {
                                 t->seams != s->tune.seams;
     s->tune = *t;
     drop_graph(s);
+    if (s->subset) return list_all(s);
     if (layout_changed) return upload_tasks(s);
     return NPDE_OK;
 }

@@ -439,6 +439,40 @@ def test_seams_host_driven_steps_match_oracle(npde_mod, oracle):
         assert same(b.get_state(0), a)
 
 
+def test_tail_retiling_keeps_every_instance_exact(npde_mod, oracle):
+    """An uneven batch whose large instances step long after the small ones have finished: the device
+    loop rebuilds its task list for the instances still stepping (new band height and layout) in the
+    middle of the run.  Every instance's state, frames and frame times equal its own single-instance run,
+    and a 256^2 instance equals the oracle."""
+    from npde.families import riemann
+    sizes = [64, 64, 256, 64, 64, 256, 64, 64]
+    ics = [np.ascontiguousarray(riemann(50 + k, n)[0]) for k, n in enumerate(sizes)]
+    grids = [npde_mod.make_grid(n, n, bc_x=1, bc_y=k % 2) for k, n in enumerate(sizes)]
+    T, NOUT = 0.03, 3
+
+    def run(gs, us, **tuning):
+        with npde_mod.Batch(gs, "hllc", "vanleer", 0.5) as b:
+            b.set_tuning(steps_per_graph=4, **tuning)
+            for k, U in enumerate(us):
+                b.set_state(k, U)
+            b.run_trajectory(T, NOUT, sizes=(32,))
+            return [(b.time(k), b.get_state(k), b.trajectory(k, 32), b.frame_times(k)) for k in range(len(gs))]
+
+    batch = run(grids, ics)
+    steps = [r[0][1] for r in batch]
+    assert max(steps) >= 3 * min(steps)   # a long tail: the 256^2 instances take several times the steps
+    for k in range(len(sizes)):
+        (t1, s1), U1, F1, ft1 = run([grids[k]], [ics[k]])[0]
+        (t0, s0), U0, F0, ft0 = batch[k]
+        assert (t0, s0) == (t1, s1), k
+        assert same(U0, U1) and same(F0, F1) and same(ft0, ft1), k
+    k = 2
+    a = ics[k].copy()
+    g = oracle2d.Grid.make(256, 256, bc_x="periodic", bc_y="transmissive")
+    ro = oracle.run(g, "hllc", "vanleer", 0.5, a, T, nout=NOUT)
+    assert batch[k][0][1] == ro["steps"] and same(batch[k][1], a)
+
+
 def test_seam_layout_is_the_default_for_wide_instances(npde_mod):
     """The default tuning runs batches of at least 2^20 cells whose widths are multiples of 64 (the
     benchmark batch among them) in seam mode, except 64-wide periodic instances (row groups of one
