@@ -7,7 +7,8 @@ per-instance euler2d processes and raw files become batched device loops writing
 directly:
 
 1. Generate.  Instances of the benchmark families (npde.families) are solved with the plain binary64
-   solver (HLLC + van Leer, CFL 0.5) in batches of up to `batch` instances of any sizes, 50 frame
+   solver (HLLC + van Leer, CFL 0.5) in batches of up to `batch` like instances (sorted by time window
+   and grid size, so that a batch's members step for similar counts), 50 frame
    intervals from the family's t_start to t_end, block-averaged on the GPU to every requested size
    (trajectory.hpp:31-51).
 2. Split by (family, index) only (data/splits.py:27-33): training families (riemann, blast) go to
@@ -74,6 +75,15 @@ def pressure(U: np.ndarray) -> np.ndarray:
     return (GAMMA - 1) * (E - 0.5 * (mx ** 2 + my ** 2) / rho)
 
 
+def instance_size(family: str, index: int, riemann_n: int) -> int:
+    """Grid size of an instance, without building its initial condition."""
+    if family == "riemann":
+        return riemann_n
+    if family == "blast":
+        return families.blast_size(index)
+    return KH_N
+
+
 def instance(family: str, index: int, riemann_n: int):
     """(U, n, t_start, t_end) of one instance."""
     if family == "riemann":
@@ -93,6 +103,9 @@ def generate(items, sizes, riemann_n=512, batch=64, log=print):
     "t": (51,), "n": n, "steps": steps}} and the seconds spent."""
     out = {}
     t0 = time.perf_counter()
+    # batches of like instances (same time window, same size) step for similar counts, so a batch does
+    # not wait long for its slowest member; results are keyed by (family, index), so order is free
+    items = sorted(items, key=lambda it: (it[0] == "kh", instance_size(it[0], it[1], riemann_n)))
     for s0 in range(0, len(items), batch):
         chunk = items[s0:s0 + batch]
         ics = [instance(f, i, riemann_n) for f, i in chunk]
