@@ -72,7 +72,7 @@ typedef struct {
     int seams;            /* when every instance's width is a multiple of 64, row groups do not apply and bands
                              are at most 64 rows, each wavefront owns a 64-column strip and first solves the
                              two strip boundaries of its band (from edge columns the previous stage kernel
-                             wrote): 1 (default) for batches of at least 2^20 cells, 2 at any size; 0: never
+                             wrote): 1 (default) for batches of at least 2^18 cells, 2 at any size; 0: never
                              (60-column strips with a 2-column halo) */
 } npde_tuning;
 

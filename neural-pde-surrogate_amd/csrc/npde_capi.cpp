@@ -157,8 +157,9 @@ int row_group_waves(const npde_solver* s) { return row_group_waves_for(s, s->tun
 // column is recomputed and no wavefront waits for another.  seams = 1 (the default) uses it for
 // batches of at least kSeamMinCells cells: the seam solves lengthen every wavefront's serial path,
 // which a batch too small to fill the GPU pays in full (a single 256^2 instance: 38 against 28 us per
-// step in strips); seams = 2 uses it at any size.
-constexpr long long kSeamMinCells = 1 << 20;
+// step in strips), while from 2^18 cells on seams are ahead (a single 512^2 instance: 37.5 against
+// 45.9 us); seams = 2 uses it at any size.
+constexpr long long kSeamMinCells = 1 << 18;
 bool seams_apply_for(const npde_solver* s, const npde_tuning& tune) {
     if (tune.seams <= 0 || s->n == 0 || row_group_waves_for(s, tune) > 0) return false;
     if (tune.rows_per_task > 64) return false;

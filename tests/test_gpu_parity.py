@@ -474,7 +474,7 @@ def test_tail_retiling_keeps_every_instance_exact(npde_mod, oracle):
 
 
 def test_seam_layout_is_the_default_for_wide_instances(npde_mod):
-    """The default tuning runs batches of at least 2^20 cells whose widths are multiples of 64 (the
+    """The default tuning runs batches of at least 2^18 cells whose widths are multiples of 64 (the
     benchmark batch among them) in seam mode, except 64-wide periodic instances (row groups of one
     wavefront); smaller batches and other widths run in 60-column strips."""
     def layout(nx, bc_x, count, **tuning):
@@ -491,6 +491,7 @@ def test_seam_layout_is_the_default_for_wide_instances(npde_mod):
     assert layout(100, 1, 256) == "strips"
     assert layout(256, 0, 1) == "strips"              # 2^14 cells: too small to fill the GPU
     assert layout(256, 0, 1, seams=2) == "seams"
+    assert layout(512, 0, 8) == "seams"               # 2^18 cells
 
 
 def test_trajectory_invariants_of_check_trajectory(npde_mod):
