@@ -119,7 +119,9 @@ bool valid_grid(const npde_grid_desc& g, std::string& why) {
 // Band height: the tuned value, or (rows_per_task <= 0) the tallest band in [4, 64] that still gives
 // about 2048 tasks over the listed instances, so that small grids are not bound by one wavefront
 // marching many rows, while bands stay tall enough not to recompute their 2 + 2 edge rows too often
-// (measured best or within 5% of it from a single 512^2 instance to 64 x 512^2).
+// (measured best or within 5% of it from a single 512^2 instance to 64 x 512^2).  When even 4-row
+// bands leave fewer than 512 tasks, the GPU is far from full and 2-row bands are faster (a single
+// 256^2 instance: 20.6 against 25.9 us per step).
 int band_rows(const npde_solver* s, int strip_cols) {
     if (s->tune.rows_per_task > 0) return s->tune.rows_per_task;
     long long rows_total = 0;
@@ -127,6 +129,7 @@ int band_rows(const npde_solver* s, int strip_cols) {
         if (s->listed[k]) rows_total += (long long)((s->grids[k].nx + strip_cols - 1) / strip_cols) * s->grids[k].ny;
     int r = 64;
     while (r > 4 && rows_total / r < 2048) r /= 2;
+    if (r == 4 && rows_total / 4 < 512) r = 2;
     return r;
 }
 
@@ -1083,7 +1086,7 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
     double best_ms = 1e30;
     int max_ny = 1;
     for (const auto& g : s->grids) max_ny = std::max(max_ny, g.ny);
-    const int rows_c[] = {4, 8, 16, 32, 64, 128};
+    const int rows_c[] = {2, 4, 8, 16, 32, 64, 128};
     const int wpb_c[] = {1, 2, 4};
     // task layouts that apply to this batch, as (row_groups, seams): row groups up to W = 8, seam
     // strips, plain strips
