@@ -198,77 +198,86 @@ def plain_rollout(indices, n, m=COARSE, batch=64):
     return np.concatenate(frames), secs
 
 
-def learned_rollout(model: FluxCorrector, indices, n=COARSE, batch=64, force_fallback=False, log=print):
-    """Coarse HIP solver with the network's increments; frames (N, F, 4, n, n) float32, seconds,
-    and the indices that fell back to the plain fine solve (whose frames replace theirs).
+def learned_rollout_states(model: FluxCorrector, U0: np.ndarray, tf, force_fallback=False):
+    """Coarse HIP solver with the network's increments from the periodic states U0 (B, 4, n, n)
+    binary64, frames at the times tf (tf[0] = 0: the states' own time); returns frames
+    (B, F, 4, n, n) float32, the seconds of the rollout and the mask of instances that failed.
 
     The network reads the solver's state in place (zero-copy view) and writes its increments into
     fixed buffers the stage kernels read.  Everything runs on the solver's stream: per frame interval,
     one replay of the captured network graph, one device loop to the frame time, the frame copy and
     the positivity / indicator check, with no host synchronisation besides the loop's own."""
     dev = torch.device("cuda")
+    B, n = U0.shape[0], U0.shape[-1]
+    nout = len(tf) - 1
+    packed = pack_stack(model.params())
+    grids = [S.make_grid(n, n, bc_x=S.PERIODIC, bc_y=S.PERIODIC)] * B
+    with S.Batch(grids, "hllc", "vanleer", 0.5) as b, torch.no_grad():
+        b.set_tuning(steps_per_graph=8)  # ~14 CFL substeps per frame interval at 64^2
+        for k in range(B):
+            b.set_state(k, U0[k])
+        stream = torch.cuda.ExternalStream(b.stream)
+        with torch.cuda.stream(stream):
+            U = b.state_view()
+            rho_max = U[:, 0].flatten(1).amax(1)
+            CX = torch.zeros(B, 4, n, n, dtype=torch.float64, device=dev)
+            CY = torch.zeros_like(CX)
+            for k in range(B):
+                b.set_flux_correction(k, CX[k].data_ptr(), CY[k].data_ptr())
+            frames = torch.empty(B, nout + 1, 4, n, n, dtype=torch.float32, device=dev)
+            frames[:, 0] = U.to(torch.float32)
+            keep = torch.ones(B, dtype=torch.bool, device=dev)
+
+            def check_and_correct():
+                # positivity / indicator check of the state just reached, then the increments of the
+                # next interval (zero for instances that failed): one captured graph per interval
+                rho = U[:, 0]
+                p = (1.4 - 1.0) * (U[:, 3] - 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2) / U[:, 0])
+                ok = (torch.isfinite(U).flatten(1).all(1) & (rho.flatten(1).amin(1) > 0)
+                      & (p.flatten(1).amin(1) > 0) & (rho.flatten(1).amax(1) <= BLOWUP * rho_max))
+                keep.logical_and_(ok)
+                model.increments_into(U, CX, CY, keep, packed)
+
+            check_and_correct()  # warm-up (allocations, kernel loads); U is the initial state
+            b.advance_to(0.0)  # warm-up: captures the solver's graph outside the timing
+            net = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(net):
+                check_and_correct()
+            keep.fill_(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            net.replay()  # increments of the first interval
+            for j in range(1, nout + 1):
+                try:
+                    b.advance_to(tf[j])
+                except NonPhysicalStateError:
+                    # every instance that stopped short of the frame time met a non-physical state;
+                    # its remaining frames come from the fine solve, and its state is reset to a
+                    # physical one so that it does not stop the following device loops again
+                    t_now, _ = b.times()
+                    for k in np.nonzero(t_now != tf[j])[0]:
+                        keep[int(k)] = False
+                        b.set_state(int(k), U0[k])
+                frames[:, j] = U.to(torch.float32)
+                if force_fallback:
+                    keep.zero_()
+                net.replay()  # check frame j, increments of interval j+1
+            torch.cuda.synchronize()
+            secs = time.perf_counter() - t0
+            return frames.cpu().numpy(), secs, (~keep).cpu().numpy()
+
+
+def learned_rollout(model: FluxCorrector, indices, n=COARSE, batch=64, force_fallback=False, log=print):
+    """Random-Riemann instances: the learned coarse solver from the block-averaged fine initial states;
+    frames (N, F, 4, n, n) float32, seconds, and the indices that fell back to the plain fine solve
+    (whose frames replace theirs; its time is counted)."""
     tf = frame_times()
     frames_all, secs, fell = [], 0.0, []
-    packed = pack_stack(model.params())
     for s0 in range(0, len(indices), batch):
         idx = list(indices[s0:s0 + batch])
-        B = len(idx)
         U0 = np.stack([block_average_f64(riemann(i, FINE)[0], n) for i in idx])
-        grids = [S.make_grid(n, n, bc_x=S.PERIODIC, bc_y=S.PERIODIC)] * B
-        with S.Batch(grids, "hllc", "vanleer", 0.5) as b, torch.no_grad():
-            b.set_tuning(steps_per_graph=8)  # ~14 CFL substeps per frame interval at 64^2
-            for k in range(B):
-                b.set_state(k, U0[k])
-            stream = torch.cuda.ExternalStream(b.stream)
-            with torch.cuda.stream(stream):
-                U = b.state_view()
-                rho_max = U[:, 0].flatten(1).amax(1)
-                CX = torch.zeros(B, 4, n, n, dtype=torch.float64, device=dev)
-                CY = torch.zeros_like(CX)
-                for k in range(B):
-                    b.set_flux_correction(k, CX[k].data_ptr(), CY[k].data_ptr())
-                frames = torch.empty(B, NOUT + 1, 4, n, n, dtype=torch.float32, device=dev)
-                frames[:, 0] = U.to(torch.float32)
-                keep = torch.ones(B, dtype=torch.bool, device=dev)
-
-                def check_and_correct():
-                    # positivity / indicator check of the state just reached, then the increments of the
-                    # next interval (zero for instances that failed): one captured graph per interval
-                    rho = U[:, 0]
-                    p = (1.4 - 1.0) * (U[:, 3] - 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2) / U[:, 0])
-                    ok = (torch.isfinite(U).flatten(1).all(1) & (rho.flatten(1).amin(1) > 0)
-                          & (p.flatten(1).amin(1) > 0) & (rho.flatten(1).amax(1) <= BLOWUP * rho_max))
-                    keep.logical_and_(ok)
-                    model.increments_into(U, CX, CY, keep, packed)
-
-                check_and_correct()  # warm-up (allocations, kernel loads); U is the initial state
-                b.advance_to(0.0)  # warm-up: captures the solver's graph outside the timing
-                net = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(net):
-                    check_and_correct()
-                keep.fill_(True)
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                net.replay()  # increments of the first interval
-                for j in range(1, NOUT + 1):
-                    try:
-                        b.advance_to(tf[j])
-                    except NonPhysicalStateError:
-                        # every instance that stopped short of the frame time met a non-physical state;
-                        # its remaining frames come from the fine solve, and its state is reset to a
-                        # physical one so that it does not stop the following device loops again
-                        t_now, _ = b.times()
-                        for k in np.nonzero(t_now != tf[j])[0]:
-                            keep[int(k)] = False
-                            b.set_state(int(k), U0[k])
-                    frames[:, j] = U.to(torch.float32)
-                    if force_fallback:
-                        keep.zero_()
-                    net.replay()  # check frame j, increments of interval j+1
-                torch.cuda.synchronize()
-                secs += time.perf_counter() - t0
-                out = frames.cpu().numpy()
-                badn = (~keep).cpu().numpy()
+        out, sec, badn = learned_rollout_states(model, U0, tf, force_fallback)
+        secs += sec
         if badn.any():
             redo = [idx[k] for k in np.nonzero(badn)[0]]
             t1 = time.perf_counter()
@@ -330,6 +339,65 @@ def evaluate(model, test_idx, truth64, ladder=(64, 128, 256), log=print):
     return res
 
 
+def evaluate_kh(model, indices, n=128, n_fine=1024, log=print):
+    """Held-out Kelvin-Helmholtz instances (npde.families.kh, never seen in training): from a random start
+    time t_s in [0.5, 1.5] to t = 2.  Truth and fallback: the plain binary64 1024^2 solve from t = 0,
+    block-averaged to n at NOUT frames after t_s (single-instance runs; their [t_s, 2] wall time is the
+    fine solver's cost; times of all runs count from t_s).  The plain and learned solvers at n start from the truth's state at t_s,
+    block-averaged in binary64, and run the same NOUT frame intervals; an instance whose learned run
+    fails takes the fine frames and is charged the fine solve's time."""
+    from ..families import kh
+    rows = []
+    for i in indices:
+        U, t_end, meta = kh(i, n_fine)
+        ts = float(meta["t_start"])
+        with S.Batch([S.make_grid(n_fine, n_fine, bc_x=S.PERIODIC, bc_y=S.PERIODIC)], "hllc", "vanleer", 0.5) as b:
+            b.set_state(0, U)
+            b.advance_to(ts)
+            fine_ts = b.get_state(0)
+            # from here on times count from t_s (set_state restarts the clock), as in the coarse runs
+            b.set_state(0, fine_ts)
+            b.run_trajectory(0.001, 1, sizes=(n,))  # warm-up: graph capture outside the timing
+            b.set_state(0, fine_ts)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.run_trajectory(t_end - ts, NOUT, sizes=(n,))
+            fine_s = time.perf_counter() - t0
+            truth = b.trajectory(0, n)[None]
+        U0 = block_average_f64(fine_ts, n)[None]
+        tf = frame_times(t_end=t_end - ts)
+        with S.Batch([S.make_grid(n, n, bc_x=S.PERIODIC, bc_y=S.PERIODIC)], "hllc", "vanleer", 0.5) as b:
+            b.set_state(0, U0[0])
+            b.run_trajectory(0.01, 1, sizes=(n,))
+            b.set_state(0, U0[0])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.run_trajectory(t_end - ts, NOUT, sizes=(n,))
+            plain_s = time.perf_counter() - t0
+            plain = b.trajectory(0, n)[None]
+        lf, learned_s, bad = learned_rollout_states(model, U0, tf)
+        fell = bool(bad[0])
+        if fell:
+            lf, learned_s = truth, learned_s + fine_s
+        r = {"index": i, "t_start": ts, "fine_seconds": fine_s, "plain_seconds": plain_s,
+             "learned_seconds": learned_s, "fallback": fell,
+             "l1_rho_plain": float(l1_density(plain, truth)[:, 1:].mean()),
+             "l1_rho_learned": float(l1_density(lf, truth)[:, 1:].mean())}
+        rows.append(r)
+        log(f"  kh {i}: t_s {ts:.3f}  L1(rho) plain {n}^2 {r['l1_rho_plain']:.5f}  learned {r['l1_rho_learned']:.5f}"
+            f"{' (fallback)' if fell else ''}  wall fine {fine_s:.3f} s, plain {plain_s:.3f} s, learned {learned_s:.3f} s")
+    mean = lambda k: float(np.mean([r[k] for r in rows]))
+    res = {"instances": rows, "n": n, "n_fine": n_fine,
+           "mean": {k: mean(k) for k in ("l1_rho_plain", "l1_rho_learned", "fine_seconds", "plain_seconds",
+                                        "learned_seconds")},
+           "fallbacks": int(sum(r["fallback"] for r in rows))}
+    m = res["mean"]
+    log(f"  kh mean: L1(rho) plain {m['l1_rho_plain']:.5f}, learned {m['l1_rho_learned']:.5f}; wall fine "
+        f"{m['fine_seconds']:.3f} s, plain {m['plain_seconds']:.3f} s, learned {m['learned_seconds']:.3f} s; "
+        f"fallbacks {res['fallbacks']}")
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
@@ -342,6 +410,8 @@ def main(argv=None):
     ap.add_argument("--init", default=None, help="start from these weights (.npz) at the full unroll")
     ap.add_argument("--out-scale", type=float, default=None, help="bound of the increments (fraction of the face "
                     "flux scale) for a new corrector")
+    ap.add_argument("--kh-eval", type=int, default=0, help="only evaluate the trained weights on this many held-out "
+                    "Kelvin-Helmholtz instances at 128^2 (kh128_eval.json)")
     args = ap.parse_args(argv)
     out = Path(args.out)
     out.mkdir(parents=True, exist_ok=True)
@@ -354,6 +424,12 @@ def main(argv=None):
         logf.flush()
 
     torch.cuda.set_device(0)
+    if args.kh_eval:
+        from .model import load_trained
+        log(f"held-out Kelvin-Helmholtz evaluation: {args.kh_eval} instances")
+        res = evaluate_kh(load_trained(args.init) if args.init else load_trained(), list(range(args.kh_eval)), log=log)
+        (out / "kh128_eval.json").write_text(json.dumps(res, indent=1))
+        return
     log(f"generating {args.train_traj} training trajectories")
     train_idx = list(range(args.train_traj))
     test_idx = list(range(100000, 100000 + args.test_traj))

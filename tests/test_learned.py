@@ -254,3 +254,21 @@ def test_trained_corrector_reduces_coarse_error(npde_mod, cuda):
     e_plain = P.l1_density(plain[own], truth[64][own])[:, 1:].mean()
     e_learned = P.l1_density(learned[own], truth[64][own])[:, 1:].mean()
     assert e_learned < 0.95 * e_plain, (e_learned, e_plain)
+
+
+@pytest.mark.gpu
+def test_held_out_kh_evaluation_runs_and_charges_fallbacks(npde_mod, cuda):
+    """The held-out Kelvin-Helmholtz protocol on a small setting (fine 256^2, coarse 64^2, two
+    instances): every instance has a truth-relative density error for both solvers, an instance that
+    fell back takes the fine frames (zero error) and is charged at least the fine solve's time."""
+    from npde.learned.model import load_trained
+    from npde.learned.pipeline import evaluate_kh
+    res = evaluate_kh(load_trained(), [0, 1], n=64, n_fine=256, log=lambda *a: None)
+    assert len(res["instances"]) == 2 and res["n"] == 64 and res["n_fine"] == 256
+    for r in res["instances"]:
+        assert 0.5 <= r["t_start"] <= 1.5
+        assert np.isfinite(r["l1_rho_plain"]) and r["l1_rho_plain"] > 0.0
+        if r["fallback"]:
+            assert r["l1_rho_learned"] == 0.0 and r["learned_seconds"] >= r["fine_seconds"]
+        else:
+            assert np.isfinite(r["l1_rho_learned"]) and r["l1_rho_learned"] > 0.0
