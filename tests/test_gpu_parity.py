@@ -287,13 +287,17 @@ def test_uneven_blast_batch_full_size(npde_mod, oracle):
 FAST_TOL_10_STEPS = 1e-12  # max |fast - exact| / max|exact| per variable after 10 steps (measured ~5e-16)
 
 
-def test_fast_arithmetic_is_opt_in_and_close_to_exact(npde_mod, oracle):
+@pytest.mark.parametrize("n,tuning,layout", [(96, {}, "strips"), (128, dict(seams=2), "seams")])
+def test_fast_arithmetic_is_opt_in_and_close_to_exact(npde_mod, oracle, n, tuning, layout):
     """NPDE_ARITH_FAST (reciprocal divisions, shortened square root, contraction) stays within
-    rounding-level distance of the exact kernels over 10 Lax-Liu steps, and switching back to
-    NPDE_ARITH_EXACT reproduces the reference bits."""
-    g, t_end, U0 = npde_mod.builtin_problem("lax_liu3", 96)
-    go, _, _ = oracle.setup("lax_liu3", 96)
+    rounding-level distance of the exact kernels over 10 Lax-Liu steps, in the strip and the seam
+    layouts, and switching back to NPDE_ARITH_EXACT reproduces the reference bits."""
+    g, t_end, U0 = npde_mod.builtin_problem("lax_liu3", n)
+    go, _, _ = oracle.setup("lax_liu3", n)
     with npde_mod.Batch([g], "hllc", "vanleer", 0.5) as b:
+        if tuning:
+            b.set_tuning(**tuning)
+        assert b.task_layout == layout
         assert b.arithmetic == "exact"
         b.set_arithmetic("fast")
         assert b.arithmetic == "fast"
@@ -312,12 +316,17 @@ def test_fast_arithmetic_is_opt_in_and_close_to_exact(npde_mod, oracle):
     assert 0.0 < dev <= FAST_TOL_10_STEPS, dev
 
 
-def test_fast_arithmetic_conserves_periodic_totals(npde_mod):
-    """The fast kernels keep the flux form: periodic totals of all four conserved variables are
-    conserved to rounding."""
+@pytest.mark.parametrize("tuning,layout", [({}, "strips"), (dict(row_groups=2), "row_groups"),
+                                           (dict(row_groups=0, seams=2), "seams")])
+def test_fast_arithmetic_conserves_periodic_totals(npde_mod, tuning, layout):
+    """The fast kernels keep the flux form in every task layout: periodic totals of all four conserved
+    variables are conserved to rounding."""
     from npde.families import riemann
     U0 = riemann(3, 128)[0]
     with npde_mod.Batch([npde_mod.make_grid(128, 128, bc_x=1, bc_y=1)], "hllc", "vanleer", 0.5) as b:
+        if tuning:
+            b.set_tuning(**tuning)
+        assert b.task_layout == layout
         b.set_arithmetic("fast")
         b.set_state(0, U0)
         b.advance_steps(40, 0.5)
