@@ -42,6 +42,10 @@ hipError_t launch_selftest_arith(const double* x, const double* y, long long n, 
 
 using namespace npde;
 
+// Captured device loops come in sizes of 1, 2, 4, ... iterations up to steps_per_graph (the last size is
+// steps_per_graph itself), one graph per size: level l runs min(2^l, steps_per_graph) iterations.
+constexpr int kGraphLevels = 32;
+
 struct npde_solver {
     int n = 0;
     int flux = kHllc, lim = kLimVanLeer;
@@ -79,8 +83,8 @@ struct npde_solver {
     // device-resident run, when the list is rebuilt for the instances still stepping (retile_tail)
     std::vector<char> listed;
     bool subset = false;
-    hipGraphExec_t graph = nullptr;
-    int graph_key_frames = -1;  // nftasks the cached graph was captured with (0: no frame kernel)
+    hipGraphExec_t graphs[kGraphLevels] = {};  // captured on first use; all share the keys below
+    int graph_key_frames = -1;  // nftasks the cached graphs were captured with (0: no frame kernel)
     int graph_key_steps = 0;
     int graph_key_corr = -1;
     int graph_key_rows = -1, graph_key_wpb = -1;
@@ -257,9 +261,54 @@ npde_status push_ctl(npde_solver* s) {
 }
 
 void drop_graph(npde_solver* s) {
-    if (s->graph) (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
+    for (auto& g : s->graphs) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
     s->graph_key_frames = -1;
+}
+
+int loop_iters(const npde_solver* s, int level) {
+    const long long K = std::max(1, s->tune.steps_per_graph);
+    return int(std::min(1LL << level, K));
+}
+
+// The device-loop size for a run that expects `steps` more steps of its slowest instance (0: unknown).
+// Each launch ends in a host round trip, and iterations past every instance's last step still launch
+// kernels that return at once, so: the next power of two when that wastes at most two iterations,
+// otherwise the largest power of two that does not overshoot (the remainder takes further launches).
+int loop_level(const npde_solver* s, long long steps) {
+    const int K = std::max(1, s->tune.steps_per_graph);
+    int top = 0;
+    while (loop_iters(s, top) < K) ++top;
+    if (steps <= 0 || steps >= K) return top;
+    int up = 0;
+    while ((1LL << up) < steps) ++up;
+    if ((1LL << up) - steps <= 2) return std::min(up, top);
+    return std::min(up - 1, top);
+}
+
+// Steps the slowest stepping instance still needs to reach its end time at its current CFL step, from the
+// host copy of the run state: fresh after a launch of this run (then each instance's own end time, step
+// budget and done flag count), or left from the previous call at the start of a run (then t_end and
+// max_new_steps are the new run's).  dt changes as the state evolves, so this only sizes launches and
+// never changes a result.  0 = unknown.
+long long expected_steps(const npde_solver* s, bool fresh, double t_end, long long max_new_steps) {
+    long long est = 0;
+    for (int k = 0; k < s->n; ++k) {
+        if (!s->listed[k]) continue;
+        const Ctl& c = s->hctl[k];
+        if (fresh && c.done) continue;
+        const InstDesc& D = s->hdesc[k];
+        const double rate = c.sx / D.dx + c.sy / D.dy;
+        if (!(rate > 0.0) || !std::isfinite(rate)) return 0;
+        const double need = std::ceil(((fresh ? c.t_end : t_end) - c.t) * rate / D.cfl);
+        long long st = need >= 1.0 ? (need < 1e12 ? (long long)need : (long long)1e12) : 1;
+        if (fresh) st = std::min(st, std::max(1LL, c.step_limit - c.steps));
+        else if (max_new_steps >= 0) st = std::min(st, std::max(1LL, max_new_steps));
+        est = std::max(est, st);
+    }
+    return est;
 }
 
 // The task list for the instances in mask (band height and layout chosen for them), and back to all.
@@ -338,34 +387,35 @@ npde_status launch_iterations(npde_solver* s, int iters, int nft) {
     return NPDE_OK;
 }
 
-npde_status ensure_graph(npde_solver* s, int nft) {
+npde_status ensure_graph(npde_solver* s, int nft, int level) {
     const int K = std::max(1, s->tune.steps_per_graph);
-    if (s->graph && s->graph_key_frames == nft && s->graph_key_steps == K && s->graph_key_corr == int(s->has_corr) &&
-        s->graph_key_rows == s->tune.rows_per_task && s->graph_key_wpb == s->tune.waves_per_block &&
-        s->graph_key_fast == int(s->fast))
-        return NPDE_OK;
-    drop_graph(s);
+    if (!(s->graph_key_frames == nft && s->graph_key_steps == K && s->graph_key_corr == int(s->has_corr) &&
+          s->graph_key_rows == s->tune.rows_per_task && s->graph_key_wpb == s->tune.waves_per_block &&
+          s->graph_key_fast == int(s->fast))) {
+        drop_graph(s);
+        s->graph_key_frames = nft;
+        s->graph_key_steps = K;
+        s->graph_key_corr = int(s->has_corr);
+        s->graph_key_rows = s->tune.rows_per_task;
+        s->graph_key_wpb = s->tune.waves_per_block;
+        s->graph_key_fast = int(s->fast);
+    }
+    if (s->graphs[level]) return NPDE_OK;
     hipGraph_t g = nullptr;
     HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    npde_status st = launch_iterations(s, K, nft);
+    npde_status st = launch_iterations(s, loop_iters(s, level), nft);
     hipError_t e = hipStreamEndCapture(s->stream, &g);
     if (st != NPDE_OK) {
         if (g) (void)hipGraphDestroy(g);
         return st;
     }
     if (e != hipSuccess) return fail(s, NPDE_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&s->graphs[level], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
-        s->graph = nullptr;
+        s->graphs[level] = nullptr;
         return fail(s, NPDE_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
     }
-    s->graph_key_frames = nft;
-    s->graph_key_steps = K;
-    s->graph_key_corr = int(s->has_corr);
-    s->graph_key_rows = s->tune.rows_per_task;
-    s->graph_key_wpb = s->tune.waves_per_block;
-    s->graph_key_fast = int(s->fast);
     return NPDE_OK;
 }
 
@@ -374,9 +424,9 @@ npde_status ensure_graph(npde_solver* s, int nft) {
 // bands).  Rebuild the task list -- band height and layout chosen again -- for the instances still
 // stepping, when their remaining work is worth a new graph: more than two graph launches at their current
 // dt.  listed_n becomes the number still stepping either way, so the next look comes when half of them
-// have finished.  Every layout gives the same bits.
-npde_status retile_tail(npde_solver* s, int nft, int& listed_n) {
-    HIPCHK(s, hipMemcpy(s->hctl.data(), s->dctl, sizeof(Ctl) * s->n, hipMemcpyDeviceToHost));
+// have finished.  Every layout gives the same bits.  The host copy of the run state is the one read
+// after the last launch; the graphs of the new list are captured when first launched.
+npde_status retile_tail(npde_solver* s, int& listed_n) {
     std::vector<char> mask(s->n, 0);
     int active = 0;
     double remaining = 0.0;
@@ -392,9 +442,7 @@ npde_status retile_tail(npde_solver* s, int nft, int& listed_n) {
     listed_n = active;
     const int K = std::max(1, s->tune.steps_per_graph);
     if (active == 0 || remaining < 2.0 * K) return NPDE_OK;
-    npde_status st = list_instances(s, mask);
-    if (st == NPDE_OK && s->tune.use_graph) st = ensure_graph(s, nft);
-    return st;
+    return list_instances(s, mask);
 }
 
 // Device-resident loop shared by npde_advance_to and npde_run_trajectory.
@@ -403,36 +451,41 @@ npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, boo
     npde_status st = list_all(s);
     if (st != NPDE_OK) return st;
     const int nft = record ? int(s->hftasks.size()) : 0;
-    if (s->tune.use_graph) {
-        if ((st = ensure_graph(s, nft)) != NPDE_OK) return st;
-    }
-    // the run state is set on the device and read back once at the end: no host round trip before the run
+    // The first launch is sized from the run state the host already holds (the end of the previous call),
+    // the later ones from the run state read back after each launch.  The run state is set on the device:
+    // no host round trip before the run.  The first launch's graph is captured on an idle stream.
+    long long expect = expected_steps(s, false, t_end, max_new_steps);
+    int level = loop_level(s, expect);
+    if (s->tune.use_graph && (st = ensure_graph(s, nft, level)) != NPDE_OK) return st;
     HIPCHK(s, launch_begin_run(s->dctl, s->n, t_start, t_end, nout, record ? 1 : 0, max_new_steps, s->dndone,
                                s->stream));
     HIPCHK(s, hipEventRecord(s->ev0, s->stream));
     if ((st = launch_wavespeed_and_finish(s)) != NPDE_OK) return st;
     int done = 0;
     int listed_n = s->n;  // instances in the current task list
-    const int K = std::max(1, s->tune.steps_per_graph);
     // A state whose wave speed is infinite gives dt = 0 and would never reach its target (the reference
     // loops forever there too); bound the loop instead.
-    const long long max_chunks = std::max(1LL, 50000000LL / K);
-    for (long long chunk = 0;; ++chunk) {
-        if (chunk >= max_chunks) return fail(s, NPDE_ERR_STATE, "run did not finish within 5e7 steps");
+    long long iters = 0;
+    for (;;) {
+        if (iters >= 50000000LL) return fail(s, NPDE_ERR_STATE, "run did not finish within 5e7 steps");
         if (s->tune.use_graph) {
-            HIPCHK(s, hipGraphLaunch(s->graph, s->stream));
-        } else if ((st = launch_iterations(s, K, nft)) != NPDE_OK) {
+            if ((st = ensure_graph(s, nft, level)) != NPDE_OK) return st;
+            HIPCHK(s, hipGraphLaunch(s->graphs[level], s->stream));
+        } else if ((st = launch_iterations(s, loop_iters(s, level), nft)) != NPDE_OK) {
             return st;
         }
+        iters += loop_iters(s, level);
         HIPCHK(s, hipMemcpyAsync(&done, s->dndone, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipMemcpyAsync(s->hctl.data(), s->dctl, sizeof(Ctl) * s->n, hipMemcpyDeviceToHost, s->stream));
         HIPCHK(s, hipStreamSynchronize(s->stream));
         if (done >= s->n) break;
-        if (2 * (s->n - done) <= listed_n && listed_n > 1 && (st = retile_tail(s, nft, listed_n)) != NPDE_OK)
+        if (2 * (s->n - done) <= listed_n && listed_n > 1 && (st = retile_tail(s, listed_n)) != NPDE_OK)
             return st;
+        expect = expected_steps(s, true, t_end, max_new_steps);
+        level = loop_level(s, expect);
     }
     HIPCHK(s, hipEventRecord(s->ev1, s->stream));
-    HIPCHK(s, hipMemcpyAsync(s->hctl.data(), s->dctl, sizeof(Ctl) * s->n, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipEventSynchronize(s->ev1));
     float ms = 0.f;
     HIPCHK(s, hipEventElapsedTime(&ms, s->ev0, s->ev1));
     s->last_ms = ms;

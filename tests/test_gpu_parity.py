@@ -131,6 +131,27 @@ def test_uneven_batch_each_instance_matches_oracle(npde_mod, oracle):
             assert same(b.get_state(k), a), k
 
 
+@pytest.mark.parametrize("spg", [1, 3, 8, 32])
+def test_short_device_loops_match_oracle(npde_mod, oracle, spg):
+    """Many short npde_advance_to calls of a few steps each (as a learned rollout makes between network
+    calls) land on the oracle's frame times with its step count and bits, whatever device-loop sizes the
+    launches take (sized from the run state: 1, 2, 4, ... iterations up to steps_per_graph)."""
+    g, _, U0 = npde_mod.builtin_problem("lax_liu3", 64)
+    go, _, _ = oracle.setup("lax_liu3", 64)
+    NF, T = 12, 0.06
+    a = U0.copy()
+    ro = oracle.run(go, "hllc", "vanleer", 0.5, a, T, nout=NF)
+    with npde_mod.Batch([g, g], "hllc", "vanleer", 0.5) as b:
+        b.set_tuning(steps_per_graph=spg)
+        b.set_state(0, U0)
+        b.set_state(1, U0)
+        for k in range(1, NF + 1):
+            b.advance_to(0.0 + (T - 0.0) * k / NF)
+        t, steps = b.time(0)
+        assert steps == ro["steps"] and t == ro["t"] == T
+        assert same(b.get_state(0), a) and same(b.get_state(1), a)
+
+
 @pytest.mark.parametrize("tuning", [dict(rows_per_task=1, waves_per_block=1), dict(rows_per_task=7, waves_per_block=2),
                                     dict(rows_per_task=64, waves_per_block=4, use_graph=0),
                                     dict(rows_per_task=200, waves_per_block=4, steps_per_graph=3)])
