@@ -173,8 +173,9 @@ npde_status npde_set_arithmetic(npde_solver* s, int arith);
 npde_status npde_get_arithmetic(const npde_solver* s, int* arith);
 /* The task layout (npde_task_layout) the current tuning gives this handle's instances. */
 npde_status npde_get_task_layout(const npde_solver* s, int* layout);
-/* Average device milliseconds of the stage-1 and stage-2 kernel launches over nsteps host-driven
- * steps of the current state (HIP events on the handle's stream around each launch; the state advances). */
+/* Average device milliseconds of the stage-1 and stage-2 kernel launches over the next nsteps steps of
+ * every instance, each with the CFL dt of its state as in npde_advance_steps, launched one at a time with
+ * HIP events on the handle's stream around each stage kernel (the state advances). */
 npde_status npde_time_stages(npde_solver* s, int nsteps, double* ms_stage1, double* ms_stage2);
 /* Milliseconds of device time of the last npde_step / npde_advance_to / npde_run_trajectory. */
 npde_status npde_last_elapsed_ms(const npde_solver* s, double* ms);

@@ -785,19 +785,20 @@ npde_status npde_advance_steps(npde_solver* s, long long nsteps, double t_target
 
 npde_status npde_time_stages(npde_solver* s, int nsteps, double* ms_stage1, double* ms_stage2) {
     if (!s || nsteps <= 0 || !ms_stage1 || !ms_stage2) return fail(s, NPDE_ERR_INVALID, "bad arguments");
-    std::vector<double> dts(s->n);
-    npde_status st = npde_compute_dt(s, dts.data(), nullptr);
+    npde_status st = list_all(s);
     if (st != NPDE_OK) return st;
-    if ((st = set_host_mode(s)) != NPDE_OK) return st;
-    HIPCHK(s, hipMemcpyAsync(s->ddts, dts.data(), sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
+    // The steps of npde_advance_steps toward an end time they never reach, so each takes the CFL dt of its
+    // own state, launched one at a time with events around the two stage kernels.  The first opens the
+    // launch sequence, as in the device loop.
+    const double t_open = 1e300;
+    HIPCHK(s, launch_begin_run(s->dctl, s->n, t_open, t_open, 0, 0, nsteps, s->dndone, s->stream));
+    if ((st = launch_wavespeed_and_finish(s)) != NPDE_OK) return st;
     hipEvent_t e[3] = {};
     hipError_t he = hipSuccess;
     for (auto& x : e)
         if (he == hipSuccess) he = hipEventCreate(&x);
     double acc[2] = {0.0, 0.0};
-    // the steps follow each other as in the device loop (the first opens the sequence)
     for (int k = 0; k < nsteps && st == NPDE_OK && he == hipSuccess; ++k) {
-        he = launch_prep_host_step(s->dctl, s->n, s->ddts, s->stream);
         for (int stage = 1; stage <= 2 && st == NPDE_OK && he == hipSuccess; ++stage) {
             he = hipEventRecord(e[stage - 1], s->stream);
             if (he == hipSuccess) st = launch_stage_kernel(s, stage, k == 0);
@@ -816,9 +817,11 @@ npde_status npde_time_stages(npde_solver* s, int nsteps, double* ms_stage1, doub
         if (x) (void)hipEventDestroy(x);
     if (st != NPDE_OK) return st;
     if (he != hipSuccess) return fail(s, NPDE_ERR_DEVICE, std::string("timing: ") + hipGetErrorString(he));
+    HIPCHK(s, hipMemcpyAsync(s->hctl.data(), s->dctl, sizeof(Ctl) * s->n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
     *ms_stage1 = acc[0] / nsteps;
     *ms_stage2 = acc[1] / nsteps;
-    return NPDE_OK;
+    return fill_bad(s, nullptr);
 }
 
 npde_status npde_run_trajectory(npde_solver* s, double t_start, double t_end, int nout, const int* sizes, int n_sizes,

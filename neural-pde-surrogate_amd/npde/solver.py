@@ -250,7 +250,8 @@ class Batch:
         check(self._h, self._lib.npde_advance_steps(self._h, int(nsteps), float(t_target), C.byref(bad)), bad)
 
     def time_stages(self, nsteps: int):
-        """Average device ms of the stage-1 and stage-2 kernels over nsteps host-driven steps."""
+        """Average device ms of the stage-1 and stage-2 kernels over the next nsteps CFL steps, launched
+        one at a time with HIP events around each (the state advances)."""
         a, b = C.c_double(), C.c_double()
         check(self._h, self._lib.npde_time_stages(self._h, int(nsteps), C.byref(a), C.byref(b)))
         return a.value, b.value

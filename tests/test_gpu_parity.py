@@ -131,6 +131,22 @@ def test_uneven_batch_each_instance_matches_oracle(npde_mod, oracle):
             assert same(b.get_state(k), a), k
 
 
+def test_stage_timing_takes_the_device_loops_steps(npde_mod):
+    """npde_time_stages advances every instance by its next CFL steps: the same time, step count and
+    bits as npde_advance_steps over as many steps."""
+    g, _, U0 = npde_mod.builtin_problem("lax_liu3", 64)
+    with npde_mod.Batch([g, g], "hllc", "vanleer", 0.5) as a, npde_mod.Batch([g, g], "hllc", "vanleer", 0.5) as b:
+        for h in (a, b):
+            h.set_state(0, U0)
+            h.set_state(1, U0)
+        ms1, ms2 = a.time_stages(25)
+        b.advance_steps(25, 1e3)
+        assert ms1 > 0 and ms2 > 0
+        for k in range(2):
+            assert a.time(k) == b.time(k) and a.time(k)[1] == 25
+            assert same(a.get_state(k), b.get_state(k))
+
+
 @pytest.mark.parametrize("spg", [1, 3, 8, 32])
 def test_short_device_loops_match_oracle(npde_mod, oracle, spg):
     """Many short npde_advance_to calls of a few steps each (as a learned rollout makes between network
