@@ -11,13 +11,15 @@ mean number of steps these 64 instances take to the workload's end time t = 0.5 
 and every instance takes exactly K timed steps (no end time stops one early), so the timed steps
 sample the states of a whole trajectory.
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-seconds S]
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-seconds S] [--no-learned]
 
 Prints one JSON line.  roofline: the stage-2 kernel (the dominant launch), whose algorithmic HBM
 traffic is 96 B per cell (reads U and U1, writes U^{n+1}); its average duration is measured with
-HIP events on the handle's stream, over a second pass through the same steps (stage_times).  cpu_baseline: the reference solver compiled from its sources
-(oracle/_ref) when that build is present, else the repo's CPU port of it, timed on the host cores
-on instance 0 of the batch.
+HIP events on the handle's stream, over a second pass through the same steps (stage_times).
+cpu_baseline: the reference solver compiled from its sources (oracle/_ref) when that build is
+present, else the repo's CPU port of it, timed on the host cores on instance 0 of the batch.
+learned_matched_l1: the metric's second half, the learned 64^2 solver's time-to-matched-L1 speed-up
+over the plain solver (learned_eval).
 """
 import argparse
 import json
@@ -100,6 +102,30 @@ def valu_roofline(layout, launch_ms):
     return out
 
 
+def learned_eval():
+    """The metric's second half, time-to-matched-L1 on the random-Riemann family
+    (npde.learned.pipeline.evaluate, experiments/solver_baseline.py:96-124): the committed flux
+    corrector on the 64^2 solver against the plain solver at 64^2, 128^2 and 256^2, on 64 held-out
+    instances (indices 100000-100063, never trained on), density L1 over 50 frames against the 512^2
+    solve block-averaged to 64^2; the matched plain resolution and its wall time by log-log
+    interpolation.  The learned wall time includes the fallback fine solves."""
+    from npde.learned import pipeline as P
+    from npde.learned.model import load_trained
+    idx = list(range(100000, 100064))
+    quiet = lambda *a: None
+    truth, _ = P.generate(idx, log=quiet)
+    res = P.evaluate(load_trained(), idx, truth[P.COARSE], log=quiet)
+    lr, m = res["learned64"], res["matched"]
+    log(f"learned 64^2: L1(rho) {lr['l1_rho']:.4f} in {lr['seconds']:.3f} s ({lr['fallbacks']} fallbacks); matched plain "
+        f"{m['plain_resolution']:.0f}^2 in {m['plain_seconds']:.3f} s: {m['speedup']:.2f}x")
+    return {"metric": "time-to-matched-L1 speed-up, learned 64^2 vs plain", "value": m["speedup"],
+            "target": 5.0, "higher_is_better": True, "learned_l1_rho": lr["l1_rho"], "learned_seconds": lr["seconds"],
+            "fallbacks": lr["fallbacks"], "matched_plain_resolution": m["plain_resolution"],
+            "matched_plain_seconds": m["plain_seconds"],
+            "plain": {str(n): v for n, v in res["plain"].items()},
+            "sample": "64 held-out random-Riemann instances, t = 0 to 0.5, 50 frames, truth 512^2"}
+
+
 def stage_times(b, ics, warmup, steps):
     """Average stage-1 and stage-2 launch durations over the same states as the timed steps: the batch
     restarts from its initial states, takes the warm-up steps, then the timed steps again one at a time
@@ -160,6 +186,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-fast", action="store_true", help="skip the fast-arithmetic (not bit-exact) variant")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-solver time-to-matched-L1 evaluation")
     ap.add_argument("--rows", type=int, default=None, help="fix the band height (skips autotuning)")
     ap.add_argument("--wpb", type=int, default=None, help="fix the waves per block (skips autotuning)")
     ap.add_argument("--row-groups", type=int, default=None, choices=(0, 1, 2),
@@ -226,6 +253,7 @@ def main():
         f"step {ms_step:.3f} ms, {value / 1e9:.2f} G cell-updates/s")
     fast = None if args.no_fast else fast_variant(b, ics, args.steps, args.warmup)
     b.close()
+    learned = None if args.no_learned else learned_eval()
 
     cpu = None if args.no_cpu else cpu_baseline(ics[0], args.cpu_seconds)
     out = {
@@ -243,6 +271,7 @@ def main():
         "valu_roofline": valu_roofline(layout, ms2),
         "cpu_baseline": cpu,
         "fast_arith_variant": fast,
+        "learned_matched_l1": learned,
     }
     print(json.dumps(out), flush=True)
 
