@@ -83,7 +83,9 @@ struct npde_solver {
     // device-resident run, when the list is rebuilt for the instances still stepping (retile_tail)
     std::vector<char> listed;
     bool subset = false;
-    hipGraphExec_t graphs[kGraphLevels] = {};  // captured on first use; all share the keys below
+    // [1]: iterations with the frame kernels of a recording run, [0]: without; captured on first use, all
+    // sharing the keys below
+    hipGraphExec_t graphs[2][kGraphLevels] = {};
     int graph_key_frames = -1;  // nftasks the cached graphs were captured with (0: no frame kernel)
     int graph_key_steps = 0;
     int graph_key_corr = -1;
@@ -261,10 +263,11 @@ npde_status push_ctl(npde_solver* s) {
 }
 
 void drop_graph(npde_solver* s) {
-    for (auto& g : s->graphs) {
-        if (g) (void)hipGraphExecDestroy(g);
-        g = nullptr;
-    }
+    for (auto& row : s->graphs)
+        for (auto& g : row) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
     s->graph_key_frames = -1;
 }
 
@@ -309,6 +312,30 @@ long long expected_steps(const npde_solver* s, bool fresh, double t_end, long lo
         est = std::max(est, st);
     }
     return est;
+}
+
+// Steps until the first stepping instance of a recording run lands on its next frame time (its target
+// T(k), main2d.cpp:224-246) at its current CFL step, from the run state read after the last launch:
+// 0 when a frame waits to be written, -1 when unknown.  Launches shorter than that can leave the frame
+// kernels out.  An instance that lands during such a launch anyway keeps its frame pending and takes no
+// step until a launch with the frame kernels writes it, so the estimate never changes a result.
+long long steps_to_frame(const npde_solver* s) {
+    long long best = -1;
+    for (int k = 0; k < s->n; ++k) {
+        if (!s->listed[k]) continue;
+        const Ctl& c = s->hctl[k];
+        if (c.done) continue;
+        if (c.pending >= 0) return 0;
+        if (!c.record || c.nout <= 0) return -1;
+        const double T = c.k == 0 ? c.t_start : c.t_start + (c.t_end - c.t_start) * double(c.k) / double(c.nout);
+        const InstDesc& D = s->hdesc[k];
+        const double rate = c.sx / D.dx + c.sy / D.dy;
+        if (!(rate > 0.0) || !std::isfinite(rate)) return -1;
+        const double need = std::ceil((T - c.t) * rate / D.cfl);
+        const long long st = need >= 1.0 ? (need < 1e12 ? (long long)need : (long long)1e12) : 1;
+        best = best < 0 ? st : std::min(best, st);
+    }
+    return best;
 }
 
 // The task list for the instances in mask (band height and layout chosen for them), and back to all.
@@ -387,7 +414,8 @@ npde_status launch_iterations(npde_solver* s, int iters, int nft) {
     return NPDE_OK;
 }
 
-npde_status ensure_graph(npde_solver* s, int nft, int level) {
+// frames: capture the iterations with the frame kernels (nft > 0) or without them.
+npde_status ensure_graph(npde_solver* s, int nft, int level, bool frames) {
     const int K = std::max(1, s->tune.steps_per_graph);
     if (!(s->graph_key_frames == nft && s->graph_key_steps == K && s->graph_key_corr == int(s->has_corr) &&
           s->graph_key_rows == s->tune.rows_per_task && s->graph_key_wpb == s->tune.waves_per_block &&
@@ -400,20 +428,21 @@ npde_status ensure_graph(npde_solver* s, int nft, int level) {
         s->graph_key_wpb = s->tune.waves_per_block;
         s->graph_key_fast = int(s->fast);
     }
-    if (s->graphs[level]) return NPDE_OK;
+    hipGraphExec_t& slot = s->graphs[frames ? 1 : 0][level];
+    if (slot) return NPDE_OK;
     hipGraph_t g = nullptr;
     HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    npde_status st = launch_iterations(s, loop_iters(s, level), nft);
+    npde_status st = launch_iterations(s, loop_iters(s, level), frames ? nft : 0);
     hipError_t e = hipStreamEndCapture(s->stream, &g);
     if (st != NPDE_OK) {
         if (g) (void)hipGraphDestroy(g);
         return st;
     }
     if (e != hipSuccess) return fail(s, NPDE_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&s->graphs[level], g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&slot, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
-        s->graphs[level] = nullptr;
+        slot = nullptr;
         return fail(s, NPDE_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
     }
     return NPDE_OK;
@@ -456,7 +485,8 @@ npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, boo
     // no host round trip before the run.  The first launch's graph is captured on an idle stream.
     long long expect = expected_steps(s, false, t_end, max_new_steps);
     int level = loop_level(s, expect);
-    if (s->tune.use_graph && (st = ensure_graph(s, nft, level)) != NPDE_OK) return st;
+    bool frames = nft > 0;  // the frame kernels in this launch
+    if (s->tune.use_graph && (st = ensure_graph(s, nft, level, frames)) != NPDE_OK) return st;
     HIPCHK(s, launch_begin_run(s->dctl, s->n, t_start, t_end, nout, record ? 1 : 0, max_new_steps, s->dndone,
                                s->stream));
     HIPCHK(s, hipEventRecord(s->ev0, s->stream));
@@ -469,9 +499,9 @@ npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, boo
     for (;;) {
         if (iters >= 50000000LL) return fail(s, NPDE_ERR_STATE, "run did not finish within 5e7 steps");
         if (s->tune.use_graph) {
-            if ((st = ensure_graph(s, nft, level)) != NPDE_OK) return st;
-            HIPCHK(s, hipGraphLaunch(s->graphs[level], s->stream));
-        } else if ((st = launch_iterations(s, loop_iters(s, level), nft)) != NPDE_OK) {
+            if ((st = ensure_graph(s, nft, level, frames)) != NPDE_OK) return st;
+            HIPCHK(s, hipGraphLaunch(s->graphs[frames ? 1 : 0][level], s->stream));
+        } else if ((st = launch_iterations(s, loop_iters(s, level), frames ? nft : 0)) != NPDE_OK) {
             return st;
         }
         iters += loop_iters(s, level);
@@ -483,6 +513,17 @@ npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, boo
             return st;
         expect = expected_steps(s, true, t_end, max_new_steps);
         level = loop_level(s, expect);
+        frames = nft > 0;
+        if (frames) {
+            // Every stepping instance is at least 8 steps from its next frame time (one stays near one
+            // almost always in a large batch, never in a single-instance run): run the steps before it
+            // without the frame kernel and its finish, 3 launches per step instead of 5.
+            const long long tf = steps_to_frame(s);
+            if (tf >= 8) {
+                frames = false;
+                level = loop_level(s, std::min(expect > 0 ? expect : tf - 2, tf - 2));
+            }
+        }
     }
     HIPCHK(s, hipEventRecord(s->ev1, s->stream));
     HIPCHK(s, hipEventSynchronize(s->ev1));

@@ -147,6 +147,31 @@ def test_stage_timing_takes_the_device_loops_steps(npde_mod):
             assert same(a.get_state(k), b.get_state(k))
 
 
+@pytest.mark.parametrize("spg", [1, 8, 32])
+def test_frames_between_frame_free_launches_match_oracle(npde_mod, oracle, spg):
+    """Recording runs launch the steps before the next frame time without the frame kernels when every
+    stepping instance is far enough from it; every frame, frame time, step count and final state still
+    equals the oracle's, for one instance and for two of different sizes (different landing steps)."""
+    from npde.families import riemann
+    NF, T = 6, 0.06
+    specs = [(oracle2d.Grid.make(64, 64, bc_x=1, bc_y=1), np.ascontiguousarray(riemann(3, 64)[0])),
+             (oracle2d.Grid.make(96, 96, bc_x=1, bc_y=1), np.ascontiguousarray(riemann(4, 96)[0]))]
+    for cases in (specs[:1], specs):
+        with npde_mod.Batch([to_npde_grid(npde_mod, g) for g, _ in cases], "hllc", "vanleer", 0.5) as b:
+            b.set_tuning(steps_per_graph=spg)
+            for k, (_, U) in enumerate(cases):
+                b.set_state(k, U)
+            b.run_trajectory(T, NF, sizes=(16,))
+            for k, (g, U) in enumerate(cases):
+                a = U.copy()
+                ro = oracle.run(g, "hllc", "vanleer", 0.5, a, T, nout=NF, sizes=(16,))
+                t, steps = b.time(k)
+                assert steps == ro["steps"] and t == ro["t"], (len(cases), k)
+                assert np.array_equal(b.frame_times(k), ro["frame_t"]), (len(cases), k)
+                assert np.array_equal(b.trajectory(k, 16), ro["frames"][16]), (len(cases), k)
+                assert same(b.get_state(k), a), (len(cases), k)
+
+
 @pytest.mark.parametrize("spg", [1, 3, 8, 32])
 def test_short_device_loops_match_oracle(npde_mod, oracle, spg):
     """Many short npde_advance_to calls of a few steps each (as a learned rollout makes between network
