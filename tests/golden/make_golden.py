@@ -10,6 +10,10 @@ headers and driver compiled from solver/src by oracle/Makefile) and records
   traj_ll64.npz    block-averaged float32 trajectory written by the reference euler2d CLI
                    (--traj --save 64,32,16, trajectory.hpp) for Lax-Liu 64^2, read back with h5dump
   cli.json         step counts and "Done:" lines of the reference CLI (Lax-Liu 256^2, nout 10 -> 682)
+  h5_layout.json   `h5dump -p -A` of a snapshot (sine2d 32^2, nout 2, snap_0001.h5) and of a trajectory
+                   file (Lax-Liu 32^2, nout 2, --save 32,16) written by the reference CLI: dataset and
+                   attribute names, types, shapes, storage and filters, attribute values
+  snap_sine32.npz  the datasets of that snapshot (rho, u, v, p, x, y)
 
 Only data is written: inputs and outputs.  Usage:  python tests/golden/make_golden.py
 """
@@ -142,6 +146,28 @@ def cli(ref_cli):
     (OUT / "cli.json").write_text(json.dumps(info, indent=1) + "\n")
 
 
+def h5_layout_text(path):
+    """`h5dump -p -A` of a file with its path replaced by its base name."""
+    out = subprocess.run([H5DUMP, "-p", "-A", str(path)], capture_output=True, text=True, check=True).stdout
+    return out.replace(str(path), Path(path).name)
+
+
+def h5_layouts(ref_cli):
+    info = {}
+    with tempfile.TemporaryDirectory() as d:
+        subprocess.run([str(ref_cli), "--test", "sine2d", "--n", "32", "--nout", "2", "--out", d],
+                       capture_output=True, text=True, check=True)
+        snap = Path(d) / "sine2d_hllc_vanleer_N32" / "snap_0001.h5"
+        info["snap_sine32"] = h5_layout_text(snap)
+        np.savez_compressed(OUT / "snap_sine32.npz",
+                            **{q: h5_dataset(snap, q, "<f8") for q in ("rho", "u", "v", "p", "x", "y")})
+        traj = Path(d) / "ll32.h5"
+        subprocess.run([str(ref_cli), "--test", "lax_liu3", "--n", "32", "--nout", "2", "--traj", str(traj),
+                        "--save", "32,16"], capture_output=True, text=True, check=True)
+        info["traj_ll32"] = h5_layout_text(traj)
+    (OUT / "h5_layout.json").write_text(json.dumps(info, indent=1) + "\n")
+
+
 def main():
     subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "-j8"], check=True)
     ref = oracle2d.Reference()
@@ -149,7 +175,8 @@ def main():
     kat(ref)
     runs(orc, ref)
     cli(oracle2d.REF_EULER2D)
-    for f in sorted(OUT.glob("*.npz")) + [OUT / "cli.json"]:
+    h5_layouts(oracle2d.REF_EULER2D)
+    for f in sorted(OUT.glob("*.npz")) + [OUT / "cli.json", OUT / "h5_layout.json"]:
         print(f"{f.name}: {f.stat().st_size / 1024:.1f} KiB")
 
 

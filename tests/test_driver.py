@@ -63,6 +63,42 @@ def test_trajectory_file_matches_reference_cli(npde_mod, tmp_path):
         assert np.array_equal(h5(out, f"U_{m}", "<f4").reshape(5, 4, m, m), t[f"U_{m}"]), m
 
 
+def layout(path):
+    """`h5dump -p -A` of a file (names, types, shapes, storage, filters, attribute values; no dataset
+    data) with its path replaced by its base name, as tests/golden/make_golden.py records it."""
+    if not H5DUMP:
+        pytest.skip("h5dump not available")
+    out = subprocess.run([H5DUMP, "-p", "-A", str(path)], capture_output=True, text=True, check=True).stdout
+    return out.replace(str(path), Path(path).name)
+
+
+def same_text(got, want):
+    import difflib
+    return got == want, "\n".join(difflib.unified_diff(want.splitlines(), got.splitlines(), "reference", "npde",
+                                                        lineterm="", n=2))
+
+
+@pytest.mark.gpu
+def test_hdf5_layouts_match_reference_writer(npde_mod, tmp_path):
+    """A snapshot (io_hdf5.hpp:56-92) and a trajectory file (trajectory.hpp:53-91) carry the same
+    datasets, attributes (names, types, values), storage layouts and filters as the reference CLI's
+    (tests/golden/h5_layout.json), and the snapshot's datasets are the reference's bit for bit."""
+    gold = json.loads((GOLDEN / "h5_layout.json").read_text())
+    p = run("--test", "sine2d", "--n", "32", "--nout", "2", "--out", tmp_path)
+    assert p.returncode == 0, p.stderr
+    snap = tmp_path / "sine2d_hllc_vanleer_N32" / "snap_0001.h5"
+    ok, diff = same_text(layout(snap), gold["snap_sine32"])
+    assert ok, diff
+    ref = np.load(GOLDEN / "snap_sine32.npz")
+    for q in ("rho", "u", "v", "p", "x", "y"):
+        assert np.array_equal(h5(snap, q, "<f8"), ref[q]), q
+    traj = tmp_path / "ll32.h5"
+    p = run("--test", "lax_liu3", "--n", "32", "--nout", "2", "--traj", traj, "--save", "32,16")
+    assert p.returncode == 0, p.stderr
+    ok, diff = same_text(layout(traj), gold["traj_ll32"])
+    assert ok, diff
+
+
 @pytest.mark.gpu
 def test_snapshot_mode_step_count_and_restart(npde_mod, oracle, tmp_path):
     """682 steps at 256^2 / nout 10 (experiments/results.md:94); snapshots are valid initial conditions
