@@ -138,3 +138,15 @@ def test_state_layout_detection():
         _layout(g, np.zeros((4, 6, 4)))
     with pytest.raises(TypeError):
         _layout(g, np.zeros((4, 4, 6), dtype=np.float32))
+
+
+def test_bench_measures_one_gpu_only():
+    """bench.py --gpus N with N > 1 exits with a message (status 2) before touching a device, and prints
+    no JSON line."""
+    import subprocess
+    import sys
+    p = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 2
+    assert "one GPU only" in p.stderr
+    assert p.stdout == ""
