@@ -43,14 +43,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(U0, seconds):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(U0, seconds, threads=None):
+    """The reference solver (or its port) on instance 0 of the batch for about `seconds`, with `threads`
+    OpenMP threads (default: all the host's)."""
     import numpy as np
     import oracle2d
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     g = oracle2d.Grid.make(N, N, bc_x="periodic", bc_y="periodic")
     if oracle2d.have_reference():
         kind, runner = "reference", oracle2d.Reference()
-        run = lambda U, k: runner.run(g, "hllc", "vanleer", 0.5, U, 0.5, nout=1, max_steps=k)
+        run = lambda U, k: runner.run(g, "hllc", "vanleer", 0.5, U, 0.5, nout=1, max_steps=k, threads=threads)
     else:
         kind, runner = "port", oracle2d.Oracle()
         run = lambda U, k: runner.run(g, "hllc", "vanleer", 0.5, U, 0.5, nout=1, max_steps=k, threads=threads)
@@ -67,7 +79,7 @@ def cpu_baseline(U0, seconds):
     value = N * N * done / elapsed
     return {"value": value, "unit": "cell-updates/s", "cores": threads, "kind": kind,
             "sample": f"instance 0 of the batch (512^2 random Riemann, HLLC + van Leer), {done} SSP-RK2 steps "
-                      f"from t = 0 in {elapsed:.2f} s on {threads} OpenMP threads"}
+                      f"from t = 0 in {elapsed:.2f} s on {threads} OpenMP threads of {cpu_model()}"}
 
 
 def counter(name, layout):
@@ -255,7 +267,13 @@ def main():
     b.close()
     learned = None if args.no_learned else learned_eval()
 
-    cpu = None if args.no_cpu else cpu_baseline(ics[0], args.cpu_seconds)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ics[0], args.cpu_seconds)
+        one = cpu_baseline(ics[0], args.cpu_seconds / 2, threads=1)
+        cpu["single_thread"] = {"value": one["value"], "sample": one["sample"]}
+        log(f"cpu baseline ({cpu['kind']}): {cpu['value'] / 1e6:.1f} M cell-updates/s on {cpu['cores']} threads, "
+            f"{one['value'] / 1e6:.2f} M on 1")
     out = {
         "metric": "FP64 cell-updates/s (HLLC+vanLeer)", "value": value, "unit": "cell-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
@@ -267,7 +285,12 @@ def main():
         "roofline": {"bound": "hbm", "kernel": f"npde::exact::k_stage<HLLC, vanLeer, 2, false, {layout}>",
                      "achieved": ach2, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach2 / HBM_PEAK_GBS,
                      "traffic": counter("hbm_bytes_per_launch", layout), "bytes_per_cell": BYTES_STAGE2, "launch_ms": ms2,
-                     "stage1_launch_ms": ms1, "stage1_achieved": ach1},
+                     "stage1_launch_ms": ms1, "stage1_achieved": ach1,
+                     # the whole step: every cell-update's algorithmic 160 B (stage 1 reads U and writes
+                     # U1, stage 2 reads U and U1 and writes U) at the measured cell-update rate
+                     "step_bytes_per_cell_update": BYTES_STAGE1 + BYTES_STAGE2,
+                     "step_achieved": value * (BYTES_STAGE1 + BYTES_STAGE2) / 1e9,
+                     "step_frac": value * (BYTES_STAGE1 + BYTES_STAGE2) / 1e9 / HBM_PEAK_GBS},
         "valu_roofline": valu_roofline(layout, ms2),
         "cpu_baseline": cpu,
         "fast_arith_variant": fast,

@@ -197,6 +197,8 @@ class Reference(_Lib):
         L.ref_run.argtypes = [C.POINTER(Grid), C.c_int, C.c_int, C.c_double, _D, C.c_double, C.c_double,
                               C.c_int, _D, _D, _D, C.c_longlong, _L, _D, _I]
         L.ref_run.restype = C.c_int
+        L.ref_set_threads.argtypes = [C.c_int]
+        L.ref_set_threads.restype = C.c_int
 
     def compute_dt(self, g, cfl, U):
         U = np.ascontiguousarray(U, dtype=np.float64)
@@ -211,7 +213,14 @@ class Reference(_Lib):
         return U
 
     def run(self, g, scheme, lim, cfl, U, t_end, t_start=0.0, nout=1, keep_states=False,
-            max_steps=10**7, record_dts=False):
+            max_steps=10**7, record_dts=False, threads=0):
+        """threads > 0: the reference's OpenMP loops use that many threads for this call."""
+        if threads > 0:
+            old = self.lib.ref_set_threads(threads)
+            try:
+                return self.run(g, scheme, lim, cfl, U, t_end, t_start, nout, keep_states, max_steps, record_dts)
+            finally:
+                self.lib.ref_set_threads(old)
         assert U.flags.c_contiguous and U.dtype == np.float64
         states = np.zeros((nout + 1,) + U.shape) if keep_states else None
         frame_t = np.zeros(nout + 1)

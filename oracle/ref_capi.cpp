@@ -7,6 +7,8 @@
 // convert to and from the reference's ghosted array-of-structs layout (grid2d.hpp:12-28).
 #include "solver2d.hpp"  // reference: solver/src/solver2d.hpp (+ grid2d, riemann2d, limiters, state2d)
 
+#include <omp.h>
+
 #include <algorithm>
 #include <cstdint>
 #include <vector>
@@ -186,3 +188,11 @@ int ref_run(const ref_grid* gd, int scheme, int lim, double cfl, double* P, doub
 }
 
 }  // extern "C"
+
+// OpenMP threads of the following calls from this thread (the reference parallelises every loop with
+// OpenMP and uses all cores by default); returns the previous setting.
+extern "C" int ref_set_threads(int n) {
+    const int old = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return old;
+}
