@@ -39,11 +39,17 @@ def riemann(index: int, n: int, family: str = "riemann_b1"):
     # quadrants: 0 bottom-left, 1 bottom-right, 2 top-left, 3 top-right; columns rho, u, v, p
     states = np.array([[rng.uniform(0.1, 2.0), rng.uniform(-1.0, 1.0), rng.uniform(-1.0, 1.0),
                         rng.uniform(0.1, 2.0)] for _ in range(4)])
+    # the quadrant of cell (i, j) is (x_i > cx) + 2 (y_j > cy); the centres increase, so each quadrant is
+    # a block and its conserved state (to_conserved of its primitives, the same operations per cell) is
+    # written once per block
     x = cell_centres(n)
-    X, Y = np.meshgrid(x, x)
-    quad = (X > cx).astype(int) + 2 * (Y > cy).astype(int)
-    rho, u, v, p = (states[quad, q] for q in range(4))
-    return to_conserved(rho, u, v, p), 0.5, {"cx": cx, "cy": cy, "states": states}
+    i0, j0 = int(np.count_nonzero(~(x > cx))), int(np.count_nonzero(~(x > cy)))
+    cons = to_conserved(*(states[:, q] for q in range(4)))  # (4 variables, 4 quadrants)
+    U = np.empty((4, n, n))
+    for q, (rows, cols) in enumerate(((slice(0, j0), slice(0, i0)), (slice(0, j0), slice(i0, n)),
+                                      (slice(j0, n), slice(0, i0)), (slice(j0, n), slice(i0, n)))):
+        U[:, rows, cols] = cons[:, q, None, None]
+    return U, 0.5, {"cx": cx, "cy": cy, "states": states}
 
 
 def _periodic_distance(X, Y, cx, cy):
@@ -54,6 +60,28 @@ def _periodic_distance(X, Y, cx, cy):
 
 def _disc(r, radius, width=0.005):
     return 0.5 * (1.0 - np.tanh((r - radius) / width))
+
+
+# numpy's tanh is exactly 1.0 from 19 on, so the smoothed disc is exactly 0 -- and the update
+# A + (v - A) * 0 leaves A unchanged -- wherever the periodic distance exceeds radius + 20 widths
+_DISC_REACH = 20 * 0.005
+
+
+def _add_disc(A, x, cx, cy, radius, value):
+    """A + (value - A) * disc, evaluated on the rows and columns within radius + _DISC_REACH of the centre
+    (in periodic distance along each axis) and left unchanged elsewhere, where the disc is exactly 0.
+    Per cell the operations are those of _disc(_periodic_distance(X, Y, cx, cy), radius)."""
+    dx = (x - cx + 0.5) % 1.0 - 0.5
+    dy = (x - cy + 0.5) % 1.0 - 0.5
+    cols = np.nonzero(np.abs(dx) < radius + _DISC_REACH)[0]
+    rows = np.nonzero(np.abs(dy) < radius + _DISC_REACH)[0]
+    if cols.size == 0 or rows.size == 0:
+        return A
+    ddx, ddy = dx[cols][None, :], dy[rows][:, None]
+    s = _disc(np.sqrt(ddx * ddx + ddy * ddy), radius)
+    block = np.ix_(rows, cols)
+    A[block] = A[block] + (value - A[block]) * s
+    return A
 
 
 BLAST_SIZES = (128, 256, 512, 1024)
@@ -70,21 +98,18 @@ def blast(index: int, n: int | None = None, family: str = "blast_b2"):
     n = blast_size(index, family) if n is None else n
     rng = rng_for(family, index)
     x = cell_centres(n)
-    X, Y = np.meshgrid(x, x)
     rho = np.ones((n, n))
     p = np.full((n, n), 0.1)
     blasts, bubbles = [], []
     for _ in range(int(rng.integers(1, 4))):
         cx, cy = rng.uniform(0, 1), rng.uniform(0, 1)
         r, pb = rng.uniform(0.03, 0.1), rng.uniform(5.0, 50.0)
-        s = _disc(_periodic_distance(X, Y, cx, cy), r)
-        p = p + (pb - p) * s
+        p = _add_disc(p, x, cx, cy, r, pb)
         blasts.append((cx, cy, r, pb))
     for _ in range(int(rng.integers(0, 4))):
         cx, cy = rng.uniform(0, 1), rng.uniform(0, 1)
         r, rb = rng.uniform(0.05, 0.15), rng.uniform(0.1, 5.0)
-        s = _disc(_periodic_distance(X, Y, cx, cy), r)
-        rho = rho + (rb - rho) * s
+        rho = _add_disc(rho, x, cx, cy, r, rb)
         bubbles.append((cx, cy, r, rb))
     z = np.zeros((n, n))
     return to_conserved(rho, z, z, p), 0.2, {"n": n, "blasts": blasts, "bubbles": bubbles}

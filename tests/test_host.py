@@ -150,3 +150,19 @@ def test_bench_measures_one_gpu_only():
     assert p.returncode == 2
     assert "one GPU only" in p.stderr
     assert p.stdout == ""
+
+
+def test_family_constructions_reproduce_recorded_instances():
+    """The initial conditions of the seeded families are bit-identical to the ones recorded in
+    tests/golden/family_hashes.json (tests/golden/make_family_hashes.py): the block-wise riemann fill
+    and the bounded blast discs change the construction's speed, not its values."""
+    import hashlib
+    import json
+    from npde import families
+    recorded = json.loads((ROOT / "tests" / "golden" / "family_hashes.json").read_text())
+    for case in recorded:
+        fn = getattr(families, case["family"])
+        U = fn(case["index"], case["n"])[0] if case["n"] is not None else fn(case["index"])[0]
+        assert list(U.shape) == case["shape"], case
+        got = hashlib.sha256(np.ascontiguousarray(U, dtype="<f8").tobytes()).hexdigest()
+        assert got == case["sha256"], case
