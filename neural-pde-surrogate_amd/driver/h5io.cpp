@@ -72,9 +72,9 @@ void attr(hid_t obj, const char* name, const std::string& v) {  // fixed-length 
     H5Awrite(a.id, ty.id, v.c_str());
 }
 
-void dataset(hid_t file, const char* name, const double* data, std::vector<hsize_t> dims) {
+void dataset(hid_t file, const char* name, const double* data, std::vector<hsize_t> dims, hid_t dcpl = H5P_DEFAULT) {
     H sp(H5Screate_simple(int(dims.size()), dims.data(), nullptr), H5Sclose);
-    H d(H5Dcreate2(file, name, H5T_NATIVE_DOUBLE, sp.id, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose);
+    H d(H5Dcreate2(file, name, H5T_NATIVE_DOUBLE, sp.id, H5P_DEFAULT, dcpl, H5P_DEFAULT), H5Dclose);
     if (d.id < 0) throw std::runtime_error(std::string("could not create dataset ") + name);
     H5Dwrite(d.id, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, data);
 }
@@ -256,11 +256,15 @@ void write_packed(const std::string& path, int N, int F, int n, const float* U, 
     if (N <= 0 || F <= 0 || n <= 0 || int(family.size()) != N) throw std::runtime_error("bad packed dataset shape");
     H f(H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose);
     if (f.id < 0) throw std::runtime_error("could not create " + path + " (does the folder exist?)");
+    // no object timestamps: the same trajectories pack into the same bytes
+    H untimed(H5Pcreate(H5P_DATASET_CREATE), H5Pclose);
+    H5Pset_obj_track_times(untimed.id, 0);
     {
         hsize_t dims[5] = {hsize_t(N), hsize_t(F), 4, hsize_t(n), hsize_t(n)};
         hsize_t chunk[5] = {1, 1, 4, hsize_t(n), hsize_t(n)};
         H sp(H5Screate_simple(5, dims, nullptr), H5Sclose);
         H pl(H5Pcreate(H5P_DATASET_CREATE), H5Pclose);
+        H5Pset_obj_track_times(pl.id, 0);
         H5Pset_chunk(pl.id, 5, chunk);
         H5Pset_shuffle(pl.id);
         H5Pset_deflate(pl.id, 1);
@@ -268,7 +272,7 @@ void write_packed(const std::string& path, int N, int F, int n, const float* U, 
         if (d.id < 0 || H5Dwrite(d.id, H5T_NATIVE_FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, U) < 0)
             throw std::runtime_error("could not write dataset U");
     }
-    dataset(f.id, "t", t, {hsize_t(N), hsize_t(F)});
+    dataset(f.id, "t", t, {hsize_t(N), hsize_t(F)}, untimed.id);
     {
         std::vector<const char*> ptrs(N);
         for (int k = 0; k < N; ++k) ptrs[k] = family[k].c_str();
@@ -277,14 +281,14 @@ void write_packed(const std::string& path, int N, int F, int n, const float* U, 
         H5Tset_cset(ty.id, H5T_CSET_UTF8);
         hsize_t dims[1] = {hsize_t(N)};
         H sp(H5Screate_simple(1, dims, nullptr), H5Sclose);
-        H d(H5Dcreate2(f.id, "family", ty.id, sp.id, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose);
+        H d(H5Dcreate2(f.id, "family", ty.id, sp.id, H5P_DEFAULT, untimed.id, H5P_DEFAULT), H5Dclose);
         if (d.id < 0 || H5Dwrite(d.id, ty.id, H5S_ALL, H5S_ALL, H5P_DEFAULT, ptrs.data()) < 0)
             throw std::runtime_error("could not write dataset family");
     }
     {
         hsize_t dims[1] = {hsize_t(N)};
         H sp(H5Screate_simple(1, dims, nullptr), H5Sclose);
-        H d(H5Dcreate2(f.id, "index", H5T_STD_I32LE, sp.id, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose);
+        H d(H5Dcreate2(f.id, "index", H5T_STD_I32LE, sp.id, H5P_DEFAULT, untimed.id, H5P_DEFAULT), H5Dclose);
         if (d.id < 0 || H5Dwrite(d.id, H5T_NATIVE_INT, H5S_ALL, H5S_ALL, H5P_DEFAULT, index) < 0)
             throw std::runtime_error("could not write dataset index");
     }

@@ -1,6 +1,6 @@
 """Dataset pipeline on the GPU: generate, split and pack in one job.
 
-  python -m npde.dataset --out DIR [--riemann 448] [--blast 96] [--kh 0] [--sizes 128,64]
+  python -m npde.dataset --out DIR [--riemann 448] [--blast 96] [--kh 0] [--sizes 128,64] [--batch 256]
 
 Replaces the reference's three steps (data/generate.py -> data/splits.py -> data/pack.py), whose
 per-instance euler2d processes and raw files become batched device loops writing the packed files
@@ -98,7 +98,7 @@ def instance(family: str, index: int, riemann_n: int):
     raise ValueError(f"unknown family '{family}'")
 
 
-def generate(items, sizes, riemann_n=512, batch=64, log=print):
+def generate(items, sizes, riemann_n=512, batch=256, log=print):
     """items: [(family, index)].  Returns {(family, index): {"frames": {m: (51, 4, m, m) float32},
     "t": (51,), "n": n, "steps": steps}} and the seconds spent."""
     out = {}
@@ -200,7 +200,9 @@ def main(argv=None):
     ap.add_argument("--blast", type=int, default=0, help="blast instances (indices 0..N-1)")
     ap.add_argument("--kh", type=int, default=0, help="kh instances (held out, 1024^2)")
     ap.add_argument("--sizes", default="128,64")
-    ap.add_argument("--batch", type=int, default=64)
+    # larger batches overlap more of their instances' tails: 160 instances took 10.8 s in batches of 64
+    # and 10.1 s in one batch of 256
+    ap.add_argument("--batch", type=int, default=256)
     args = ap.parse_args(argv)
     out = Path(args.out)
     out.mkdir(parents=True, exist_ok=True)

@@ -75,6 +75,20 @@ def test_packed_file_layout_round_trip(tmp_path):
     assert '"train"' in attrs and "(0): 8" in attrs and "(0): 5" in attrs
 
 
+def test_packed_files_are_deterministic(tmp_path):
+    """The same trajectories pack into the same bytes: the writer stores no object timestamps."""
+    import filecmp
+    import time
+    from npde import dataset as D
+    U = np.random.default_rng(1).random((3, 5, 4, 8, 8)).astype(np.float32)
+    t = np.tile(np.linspace(0.0, 0.5, 5), (3, 1))
+    for k in range(2):
+        D.write_packed(tmp_path / f"p{k}.h5", U, t, ["riemann"] * 3, [0, 1, 2], "train", 0.125)
+        if k == 0:
+            time.sleep(1.1)  # HDF5 timestamps have one-second resolution
+    assert filecmp.cmp(tmp_path / "p0.h5", tmp_path / "p1.h5", shallow=False)
+
+
 @pytest.mark.gpu
 def test_pipeline_end_to_end(npde_mod, tmp_path):
     """Ten riemann instances at 64^2 packed at 32^2 and 16^2: splits 8/1/1, every check passes, and the
