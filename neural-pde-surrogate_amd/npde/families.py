@@ -52,12 +52,6 @@ def riemann(index: int, n: int, family: str = "riemann_b1"):
     return U, 0.5, {"cx": cx, "cy": cy, "states": states}
 
 
-def _periodic_distance(X, Y, cx, cy):
-    dx = (X - cx + 0.5) % 1.0 - 0.5
-    dy = (Y - cy + 0.5) % 1.0 - 0.5
-    return np.sqrt(dx * dx + dy * dy)
-
-
 def _disc(r, radius, width=0.005):
     return 0.5 * (1.0 - np.tanh((r - radius) / width))
 
@@ -68,9 +62,10 @@ _DISC_REACH = 20 * 0.005
 
 
 def _add_disc(A, x, cx, cy, radius, value):
-    """A + (value - A) * disc, evaluated on the rows and columns within radius + _DISC_REACH of the centre
-    (in periodic distance along each axis) and left unchanged elsewhere, where the disc is exactly 0.
-    Per cell the operations are those of _disc(_periodic_distance(X, Y, cx, cy), radius)."""
+    """A + (value - A) * _disc(d, radius) with d the periodic distance of each cell centre from (cx, cy):
+    d = sqrt(dx*dx + dy*dy), dx = (x - cx + 0.5) % 1 - 0.5 and likewise dy (reference:
+    data/families.py:70-79).  Evaluated on the rows and columns within radius + _DISC_REACH of the
+    centre along each axis and left unchanged elsewhere, where the disc is exactly 0."""
     dx = (x - cx + 0.5) % 1.0 - 0.5
     dy = (x - cy + 0.5) % 1.0 - 0.5
     cols = np.nonzero(np.abs(dx) < radius + _DISC_REACH)[0]
