@@ -131,6 +131,33 @@ def test_uneven_batch_each_instance_matches_oracle(npde_mod, oracle):
             assert same(b.get_state(k), a), k
 
 
+@pytest.mark.parametrize("tuning", [{}, dict(rows_per_task=1, waves_per_block=1)])
+def test_grids_narrower_than_the_stencil_match_oracle(npde_mod, oracle, tuning):
+    """Grids of 1-5 cells per side (every ghost of the stencil wraps or clamps onto one or two interior
+    cells), mixed periodic and transmissive, in one batch and one by one: each equals its oracle run
+    (pinned to the reference build on the same cases by the oracle tests)."""
+    from test_oracle import TINY_GRIDS, tiny_state
+    cases = [(oracle2d.Grid.make(nx, ny, y=(0.0, ny / nx), bc_x=bx, bc_y=by), tiny_state(nx, ny, 7 * nx + ny))
+             for nx, ny, bx, by in TINY_GRIDS]
+    expect = []
+    for g, U in cases:
+        a = U.copy()
+        ro = oracle.run(g, "hllc", "vanleer", 0.5, a, 0.05, nout=2)
+        expect.append((ro, a))
+    for batch in ([list(range(len(cases)))] + [[k] for k in range(len(cases))]):
+        with npde_mod.Batch([to_npde_grid(npde_mod, cases[k][0]) for k in batch], "hllc", "vanleer", 0.5) as b:
+            if tuning:
+                b.set_tuning(**tuning)
+            for j, k in enumerate(batch):
+                b.set_state(j, cases[k][1])
+            b.run_trajectory(0.05, 2, sizes=())
+            for j, k in enumerate(batch):
+                ro, a = expect[k]
+                t, steps = b.time(j)
+                assert steps == ro["steps"] and t == ro["t"], (batch, k)
+                assert same(b.get_state(j), a), (batch, k)
+
+
 def test_stage_timing_takes_the_device_loops_steps(npde_mod):
     """npde_time_stages advances every instance by its next CFL steps: the same time, step count and
     bits as npde_advance_steps over as many steps."""

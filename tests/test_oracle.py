@@ -117,6 +117,35 @@ def test_against_reference_build_random_riemann(oracle, reference, scheme, lim):
     assert same(a, b)
 
 
+TINY_GRIDS = [(1, 1, "periodic", "periodic"), (1, 5, "transmissive", "periodic"), (5, 1, "periodic", "transmissive"),
+              (2, 3, "transmissive", "transmissive"), (3, 2, "periodic", "periodic"), (4, 4, "periodic", "transmissive"),
+              (5, 7, "transmissive", "periodic")]
+
+
+def tiny_state(nx, ny, seed):
+    """A positive, non-uniform state on an nx x ny grid (conserved planes)."""
+    from npde.solver import to_conserved
+    rng = np.random.default_rng(seed)
+    rho, u, v, p = (rng.uniform(lo, hi, (ny, nx)) for lo, hi in ((0.2, 2.0), (-0.5, 0.5), (-0.5, 0.5), (0.2, 2.0)))
+    return np.ascontiguousarray(to_conserved(rho, u, v, p))
+
+
+@pytest.mark.parametrize("nx,ny,bx,by", TINY_GRIDS)
+def test_grids_narrower_than_the_stencil_against_reference_build(oracle, reference, nx, ny, bx, by):
+    """Grids of 1-5 cells per side, where every ghost of the 5-point stencil wraps or clamps onto one or two
+    interior cells (grid2d.hpp:43-66 fills x ghosts k = 1, 2 in order, so a 1-cell periodic grid copies
+    its only cell twice): the oracle's run equals the reference build's, step by step."""
+    g = oracle2d.Grid.make(nx, ny, y=(0.0, ny / nx), bc_x=bx, bc_y=by)
+    U0 = tiny_state(nx, ny, 7 * nx + ny)
+    a, b = U0.copy(), U0.copy()
+    ra = oracle.run(g, "hllc", "vanleer", 0.5, a, 0.05, nout=2, record_dts=True)
+    rb = reference.run(g, "hllc", "vanleer", 0.5, b, 0.05, nout=2, record_dts=True)
+    assert ra["rc"] == rb["rc"] == 0
+    assert ra["steps"] == rb["steps"] > 0
+    assert same(ra["dts"], rb["dts"])
+    assert same(a, b)
+
+
 def test_against_reference_build_pointwise_random(oracle, reference):
     rng = np.random.default_rng(5)
     n = 20000
