@@ -158,6 +158,25 @@ def test_grids_narrower_than_the_stencil_match_oracle(npde_mod, oracle, tuning):
                 assert same(b.get_state(j), a), (batch, k)
 
 
+def test_anisotropic_cells_match_oracle(npde_mod, oracle):
+    """Cells with dy != dx on offset domains (the cases the oracle tests pin to the reference build),
+    batched: each instance equals its oracle run."""
+    from npde.families import riemann
+    from test_oracle import ANISOTROPIC
+    cases = [(oracle2d.Grid.make(nx, ny, x=xr, y=yr, bc_x=bx, bc_y=by),
+              np.ascontiguousarray(riemann(11, max(nx, ny))[0][:, :ny, :nx])) for nx, ny, xr, yr, bx, by in ANISOTROPIC]
+    with npde_mod.Batch([to_npde_grid(npde_mod, g) for g, _ in cases], "hllc", "vanleer", 0.5) as b:
+        for k, (_, U) in enumerate(cases):
+            b.set_state(k, U)
+        b.run_trajectory(0.04, 2, sizes=())
+        for k, (g, U) in enumerate(cases):
+            a = U.copy()
+            ro = oracle.run(g, "hllc", "vanleer", 0.5, a, 0.04, nout=2)
+            t, steps = b.time(k)
+            assert steps == ro["steps"] and t == ro["t"], k
+            assert same(b.get_state(k), a), k
+
+
 def test_stage_timing_takes_the_device_loops_steps(npde_mod):
     """npde_time_stages advances every instance by its next CFL steps: the same time, step count and
     bits as npde_advance_steps over as many steps."""

@@ -146,6 +146,26 @@ def test_grids_narrower_than_the_stencil_against_reference_build(oracle, referen
     assert same(a, b)
 
 
+ANISOTROPIC = [(32, 24, (0.0, 1.0), (0.0, 2.25), "periodic", "transmissive"),    # dy = 3 dx
+               (40, 40, (-1.0, 1.0), (0.5, 1.0), "transmissive", "periodic")]    # dy = dx / 4
+
+
+@pytest.mark.parametrize("nx,ny,xr,yr,bx,by", ANISOTROPIC)
+def test_anisotropic_cells_against_reference_build(oracle, reference, nx, ny, xr, yr, bx, by):
+    """Cells with dy != dx on offset domains: the x and y flux divergences and the CFL dt
+    (solver2d.hpp:67-83, 129-180) weigh each direction by its own spacing."""
+    from npde.families import riemann
+    g = oracle2d.Grid.make(nx, ny, x=xr, y=yr, bc_x=bx, bc_y=by)
+    U0 = np.ascontiguousarray(riemann(11, max(nx, ny))[0][:, :ny, :nx])
+    a, b = U0.copy(), U0.copy()
+    ra = oracle.run(g, "hllc", "vanleer", 0.5, a, 0.04, nout=2, record_dts=True)
+    rb = reference.run(g, "hllc", "vanleer", 0.5, b, 0.04, nout=2, record_dts=True)
+    assert ra["rc"] == rb["rc"] == 0
+    assert ra["steps"] == rb["steps"] > 0
+    assert same(ra["dts"], rb["dts"])
+    assert same(a, b)
+
+
 def test_against_reference_build_pointwise_random(oracle, reference):
     rng = np.random.default_rng(5)
     n = 20000
