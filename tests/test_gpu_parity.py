@@ -301,6 +301,50 @@ def test_sod_transpose_is_exact_across_layouts(npde_mod):
     assert np.abs(X["U"] - np.transpose(Y["U"], (0, 2, 1))[[0, 2, 1, 3]]).max() == 0.0
 
 
+@pytest.mark.parametrize("tuning", [{}, dict(rows_per_task=3, waves_per_block=2, seams=0)])
+def test_reference_solver_invariants(npde_mod, tuning):
+    """test_solver2d.cpp's invariants on the GPU kernels, with its grids, states and bars:
+    a uniform moving state on a periodic 16 x 12 grid stays uniform to 1e-13 after t = 0.5 (:36-53);
+    Sod along x on 200 x 4 and along y on 4 x 200 (:56-67) take the same step count (:71), every row of
+    the x-run is identical (:73-78), its y-momentum stays exactly 0 (:80-84), the y-run is the x-run
+    transposed with momenta swapped exactly (:86-95), mass and energy are conserved to 1e-13 (:97-101),
+    and the plateau at x = 0.78 is within 1e-3 of the exact rho*, p*, u* (:103-108)."""
+    from npde.solver import to_conserved, to_primitive
+
+    def run(g, U, t_end):
+        with npde_mod.Batch([g], "hllc", "vanleer", 0.5) as b:
+            if tuning:
+                b.set_tuning(**tuning)
+            b.set_state(0, U)
+            b.advance_to(t_end)
+            return b.time(0)[1], b.get_state(0)
+
+    g = npde_mod.make_grid(16, 12, 0.0, 1.0, 0.0, 1.0, bc_x=1, bc_y=1)
+    U0 = np.broadcast_to(to_conserved(1.3, 0.7, -0.4, 2.0)[:, None, None], (4, 12, 16)).copy()
+    _, U = run(g, U0, 0.5)
+    assert np.abs(U - U0).max() < 1e-13
+
+    N, M = 200, 4
+    gx = npde_mod.make_grid(N, M, 0.0, 1.0, 0.0, M / N)
+    gy = npde_mod.make_grid(M, N, 0.0, M / N, 0.0, 1.0)
+    left, right = to_conserved(1.0, 0.0, 0.0, 1.0), to_conserved(0.125, 0.0, 0.0, 0.1)
+    xc = (np.arange(N) + 0.5) * (1.0 / N)
+    col = np.where(xc[None, :] < 0.5, left[:, None], right[:, None])           # (4, N)
+    Ux0 = np.ascontiguousarray(np.broadcast_to(col[:, None, :], (4, M, N)))
+    Uy0 = np.ascontiguousarray(np.broadcast_to(col[:, :, None], (4, N, M)))
+    nsx, Ux = run(gx, Ux0, 0.2)
+    nsy, Uy = run(gy, Uy0, 0.2)
+    assert nsx == nsy
+    assert (Ux[0, 1:] == Ux[0, :1]).all()
+    assert np.abs(Ux[2]).max() == 0.0
+    assert np.abs(Ux - np.transpose(Uy, (0, 2, 1))[[0, 2, 1, 3]]).max() == 0.0
+    cell = (1.0 / N) * (M / N / M)
+    for q in (0, 3):
+        assert abs(Ux[q].sum() * cell - Ux0[q].sum() * cell) < 1e-13
+    rho, u, _, p = to_primitive(Ux[:, 0, int(0.78 * N)])
+    assert abs(rho - 0.26557) < 1e-3 and abs(p - 0.30313) < 1e-3 and abs(u - 0.92745) < 1e-3
+
+
 def test_lax_liu_256_published_step_count(npde_mod):
     g, t_end, U0 = npde_mod.builtin_problem("lax_liu3", 256)
     with npde_mod.Batch([g], "hllc", "vanleer", 0.5) as b:
