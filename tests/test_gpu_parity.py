@@ -345,6 +345,43 @@ def test_reference_solver_invariants(npde_mod, tuning):
     assert abs(rho - 0.26557) < 1e-3 and abs(p - 0.30313) < 1e-3 and abs(u - 0.92745) < 1e-3
 
 
+def test_trajectory_file_invariants(npde_mod):
+    """tests/check_trajectory.py's checks on a periodic 128^2 random-Riemann trajectory at four levels:
+    shapes and equally spaced frame times (:45-49); every coarse level the block average of the finest to
+    1e-6 relative (:51-55); totals equal across levels (:57-61) and constant in time (:62-67); density and
+    pressure positive (:69-74); and the finest level equal to the states of a frame-by-frame run
+    (:76-85; here exactly, as float32 of the binary64 state)."""
+    from npde.families import riemann
+    from npde.solver import to_primitive
+    n, nout, t_end, sizes = 128, 8, 0.2, (128, 64, 32, 16)
+    g = npde_mod.make_grid(n, n, bc_x=1, bc_y=1)
+    U0 = riemann(21, n)[0]
+    with npde_mod.Batch([g], "hllc", "vanleer", 0.5) as b:
+        b.set_state(0, U0)
+        b.run_trajectory(t_end, nout, sizes=sizes)
+        levels = {m: b.trajectory(0, m).astype(np.float64) for m in sizes}
+        t = b.frame_times(0)
+    assert np.allclose(t, np.linspace(0.0, t_end, nout + 1), rtol=0, atol=1e-12)
+    fine = levels[n]
+    for m in sizes:
+        assert levels[m].shape == (nout + 1, 4, m, m)
+        f = n // m
+        avg = fine.reshape(nout + 1, 4, m, f, m, f).mean(axis=(3, 5))
+        assert np.abs(avg - levels[m]).max() / np.abs(fine).max() < 1e-6, m
+    totals = {m: levels[m].mean(axis=(-2, -1)) for m in sizes}
+    for m in sizes:
+        assert np.abs(totals[m] - totals[n]).max() / np.abs(totals[n]).max() < 1e-6, m
+        rho, _, _, p = to_primitive(np.moveaxis(levels[m], 1, 0))
+        assert (rho > 0).all() and (p > 0).all(), m
+    drift = np.abs(totals[n] - totals[n][:1]).max(axis=0) / np.abs(totals[n][0]).clip(1e-12)
+    assert (drift[[0, 3]] < 1e-6).all()
+    with npde_mod.Batch([g], "hllc", "vanleer", 0.5) as b:
+        b.set_state(0, U0)
+        for k in range(nout + 1):
+            b.advance_to(0.0 + (t_end - 0.0) * k / nout)
+            assert np.array_equal(b.get_state(0).astype(np.float32), levels[n][k].astype(np.float32)), k
+
+
 def test_lax_liu_256_published_step_count(npde_mod):
     g, t_end, U0 = npde_mod.builtin_problem("lax_liu3", 256)
     with npde_mod.Batch([g], "hllc", "vanleer", 0.5) as b:
