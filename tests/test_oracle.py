@@ -166,6 +166,25 @@ def test_anisotropic_cells_against_reference_build(oracle, reference, nx, ny, xr
     assert same(a, b)
 
 
+def test_thread_count_does_not_change_bits(oracle, reference):
+    """tests/compare_runs.py:11-22's bar across OpenMP thread counts: the oracle with 1, 3 and 8 threads and
+    the reference build with 1 thread and with all of them give the same dt sequence and state bits."""
+    from npde.families import riemann
+    g = oracle2d.Grid.make(48, 40, y=(0.0, 40 / 48), bc_x="periodic", bc_y="transmissive")
+    U0 = np.ascontiguousarray(riemann(9, 48)[0][:, :40, :])
+    runs = []
+    for th in (1, 3, 8):
+        a = U0.copy()
+        r = oracle.run(g, "hllc", "vanleer", 0.5, a, 0.03, nout=2, record_dts=True, threads=th)
+        runs.append((r["steps"], r["dts"], a))
+    for th in (1, 0):
+        a = U0.copy()
+        r = reference.run(g, "hllc", "vanleer", 0.5, a, 0.03, nout=2, record_dts=True, threads=th)
+        runs.append((r["steps"], r["dts"], a))
+    for steps, dts, a in runs[1:]:
+        assert steps == runs[0][0] and same(dts, runs[0][1]) and same(a, runs[0][2])
+
+
 def test_against_reference_build_pointwise_random(oracle, reference):
     rng = np.random.default_rng(5)
     n = 20000
