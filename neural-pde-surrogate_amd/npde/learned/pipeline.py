@@ -267,12 +267,15 @@ def learned_rollout_states(model: FluxCorrector, U0: np.ndarray, tf, force_fallb
             return frames.cpu().numpy(), secs, (~keep).cpu().numpy()
 
 
-def learned_rollout(model: FluxCorrector, indices, n=COARSE, batch=64, force_fallback=False, log=print):
+def learned_rollout(model: FluxCorrector, indices, n=COARSE, batch=64, force_fallback=False, log=print,
+                    timing=None):
     """Random-Riemann instances: the learned coarse solver from the block-averaged fine initial states;
     frames (N, F, 4, n, n) float32, seconds, and the indices that fell back to the plain fine solve
-    (whose frames replace theirs; its time is counted)."""
+    (whose frames replace theirs; its time is counted).  A dict passed as `timing` receives the seconds
+    of the fallback solves alone under "fallback"."""
     tf = frame_times()
     frames_all, secs, fell = [], 0.0, []
+    fallback_secs = 0.0
     for s0 in range(0, len(indices), batch):
         idx = list(indices[s0:s0 + batch])
         U0 = np.stack([block_average_f64(riemann(i, FINE)[0], n) for i in idx])
@@ -283,12 +286,15 @@ def learned_rollout(model: FluxCorrector, indices, n=COARSE, batch=64, force_fal
             t1 = time.perf_counter()
             fine, _ = generate(redo, sizes=(n,), log=lambda *a: None)
             secs += time.perf_counter() - t1
+            fallback_secs += time.perf_counter() - t1
             for r, k in enumerate(np.nonzero(badn)[0]):
                 out[k] = fine[n][r]
             fell += redo
             log(f"  fallback to the fine solve for {len(redo)} instance(s): {redo[:8]} "
                 f"({time.perf_counter() - t1:.3f} s of the total)")
         frames_all.append(out)
+    if timing is not None:
+        timing["fallback"] = fallback_secs
     return np.concatenate(frames_all), secs, fell
 
 
@@ -320,12 +326,13 @@ def evaluate(model, test_idx, truth64, ladder=(64, 128, 256), log=print):
         e = float(l1_density(fr, truth64)[:, 1:].mean())
         res["plain"][n] = {"l1_rho": e, "seconds": secs}
         log(f"  plain {n:4d}^2: L1(rho) {e:.5f}  wall {secs:.3f} s")
-    fr, secs, fell = learned_rollout(model, test_idx, log=log)
+    timing = {}
+    fr, secs, fell = learned_rollout(model, test_idx, log=log, timing=timing)
     e = float(l1_density(fr, truth64)[:, 1:].mean())
     own = np.array([i not in fell for i in test_idx])
     e_own = float(l1_density(fr[own], truth64[own])[:, 1:].mean())
     e_plain_own = float(l1_density(plain64[own], truth64[own])[:, 1:].mean())
-    res["learned64"] = {"l1_rho": e, "seconds": secs, "fallbacks": len(fell),
+    res["learned64"] = {"l1_rho": e, "seconds": secs, "fallbacks": len(fell), "fallback_seconds": timing["fallback"],
                         "l1_rho_without_fallback_instances": e_own,
                         "plain64_l1_rho_same_instances": e_plain_own}
     log(f"  learned 64^2: L1(rho) {e:.5f}  wall {secs:.3f} s  fallbacks {len(fell)}  "
