@@ -162,7 +162,9 @@ int npde_mfma_probe(const float* a, const float* b, float* d, void* stream);
 npde_status npde_selftest_arith(const double* x, const double* y, long long n, double* q_fast, double* q_ieee,
                                 double* root);
 
-/* ---- performance ---- */
+/* ---- performance: the timing calls replace the reference's per-phase Profile (solver2d.hpp:29-52,
+ * printed by main2d.cpp:257-261); the tuning calls have no counterpart (the reference's OpenMP loops
+ * have no launch shape) ---- */
 npde_status npde_get_tuning(const npde_solver* s, npde_tuning* t);
 npde_status npde_set_tuning(npde_solver* s, const npde_tuning* t);
 /* Time candidate launch configurations on this handle's own instances (state is saved and restored)
