@@ -177,6 +177,52 @@ def test_anisotropic_cells_match_oracle(npde_mod, oracle):
             assert same(b.get_state(k), a), k
 
 
+def random_case(rng):
+    """One seeded configuration: grid (some widths seam- or row-group-eligible), boundaries, scheme,
+    limiter, launch shape, and a four-quadrant initial state with random corner and states."""
+    from npde.solver import to_conserved
+    nx = int(rng.choice([1, 3, 17, 60, 64, 100, 128, 131]))
+    ny = int(rng.choice([1, 2, 9, 33, 64, 96]))
+    bx, by = int(rng.integers(0, 2)), int(rng.integers(0, 2))
+    scheme = str(rng.choice(["rusanov", "hllc"]))
+    lim = str(rng.choice(["none", "unlimited", "minmod", "vanleer"]))
+    tuning = dict(rows_per_task=int(rng.choice([0, 1, 3, 8, 64])), waves_per_block=int(rng.choice([1, 2, 4])),
+                  seams=int(rng.choice([0, 2])), row_groups=int(rng.choice([0, 1, 2])),
+                  steps_per_graph=int(rng.choice([1, 4, 32])), use_graph=int(rng.choice([0, 1])))
+    cx, cy = int(rng.integers(0, nx + 1)), int(rng.integers(0, ny + 1))
+    W = np.stack([rng.uniform(0.1, 2.0, 4), rng.uniform(-1.0, 1.0, 4), rng.uniform(-1.0, 1.0, 4),
+                  rng.uniform(0.1, 2.0, 4)], axis=1)
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    quad = (ii >= cx).astype(int) + 2 * (jj >= cy).astype(int)
+    U = np.ascontiguousarray(to_conserved(*(W[quad, q] for q in range(4))))
+    g = oracle2d.Grid.make(nx, ny, y=(0.0, ny / nx), bc_x=bx, bc_y=by)
+    return g, U, scheme, lim, tuning
+
+
+@pytest.mark.parametrize("seed", range(64))
+def test_random_configurations_match_oracle(npde_mod, oracle, seed):
+    """Seeded random configurations over grid size, boundaries, scheme, limiter and launch shape (task
+    layout, band height, workgroup size, graph length, graphs or stream launches): the GPU run equals the
+    oracle's -- step count, time and bits, or the same first non-physical cell when the run fails."""
+    g, U, scheme, lim, tuning = random_case(np.random.default_rng(1000 + seed))
+    T = 0.03
+    a = U.copy()
+    ro = oracle.run(g, scheme, lim, 0.5, a, T, nout=3)
+    with npde_mod.Batch([to_npde_grid(npde_mod, g)], scheme, lim, 0.5) as b:
+        b.set_tuning(**tuning)
+        b.set_state(0, U)
+        if ro["bad"] is None:
+            b.run_trajectory(T, 3, sizes=())
+            t, steps = b.time(0)
+            assert steps == ro["steps"] and t == ro["t"], (seed, b.task_layout)
+            assert same(b.get_state(0), a), (seed, b.task_layout)
+        else:
+            with pytest.raises(npde_mod.NonPhysicalStateError) as e:
+                b.run_trajectory(T, 3, sizes=())
+            assert (e.value.i, e.value.j) == ro["bad"], seed
+            assert b.time(0)[1] == ro["steps"], seed
+
+
 def test_stage_timing_takes_the_device_loops_steps(npde_mod):
     """npde_time_stages advances every instance by its next CFL steps: the same time, step count and
     bits as npde_advance_steps over as many steps."""
