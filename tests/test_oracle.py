@@ -185,6 +185,23 @@ def test_thread_count_does_not_change_bits(oracle, reference):
         assert steps == runs[0][0] and same(dts, runs[0][1]) and same(a, runs[0][2])
 
 
+def test_random_configurations_against_reference_build(oracle, reference):
+    """The 64 seeded configurations of the GPU test (grid, boundaries, scheme, limiter; launch shapes do
+    not apply on the CPU): the oracle's run equals the reference build's -- dt sequence, step count and
+    bits, or the same first non-physical cell."""
+    from test_gpu_parity import random_case
+    for seed in range(64):
+        g, U, scheme, lim, _ = random_case(np.random.default_rng(1000 + seed))
+        a, b = U.copy(), U.copy()
+        ra = oracle.run(g, scheme, lim, 0.5, a, 0.03, nout=3, record_dts=True)
+        rb = reference.run(g, scheme, lim, 0.5, b, 0.03, nout=3, record_dts=True)
+        assert ra["rc"] == rb["rc"] and ra["steps"] == rb["steps"], seed
+        assert ra["bad"] == rb["bad"], seed
+        assert same(ra["dts"], rb["dts"]), seed
+        if ra["bad"] is None:
+            assert same(a, b), seed
+
+
 def test_against_reference_build_pointwise_random(oracle, reference):
     rng = np.random.default_rng(5)
     n = 20000
