@@ -223,6 +223,32 @@ def test_random_configurations_match_oracle(npde_mod, oracle, seed):
             assert b.time(0)[1] == ro["steps"], seed
 
 
+FAST_TOL_RANDOM = 1e-9  # max |fast - exact| / max|exact| per variable after the configuration's steps
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_configurations_fast_arithmetic_stays_close(npde_mod, oracle, seed):
+    """The fast-arithmetic kernels on the seeded random configurations that the exact run completes:
+    after as many CFL steps as the exact run takes, every conserved variable is within FAST_TOL_RANDOM of
+    the exact result, relative to its largest magnitude."""
+    g, U, scheme, lim, tuning = random_case(np.random.default_rng(1000 + seed))
+    ro = oracle.run(g, scheme, lim, 0.5, U.copy(), 0.03, nout=3)
+    if ro["bad"] is not None:
+        return  # the exact run fails; the error path is covered by the exact test
+    out = {}
+    for mode in ("exact", "fast"):
+        with npde_mod.Batch([to_npde_grid(npde_mod, g)], scheme, lim, 0.5) as b:
+            b.set_tuning(**tuning)
+            b.set_arithmetic(mode)
+            b.set_state(0, U)
+            b.advance_steps(ro["steps"], 1e3)
+            assert b.time(0)[1] == ro["steps"]
+            out[mode] = b.get_state(0)
+    scale = np.abs(out["exact"]).reshape(4, -1).max(axis=1).reshape(4, 1, 1)
+    dev = float(np.max(np.abs(out["fast"] - out["exact"]) / scale))
+    assert dev <= FAST_TOL_RANDOM, (seed, dev)
+
+
 def test_stage_timing_takes_the_device_loops_steps(npde_mod):
     """npde_time_stages advances every instance by its next CFL steps: the same time, step count and
     bits as npde_advance_steps over as many steps."""
