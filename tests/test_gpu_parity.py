@@ -223,6 +223,38 @@ def test_random_configurations_match_oracle(npde_mod, oracle, seed):
             assert b.time(0)[1] == ro["steps"], seed
 
 
+@pytest.mark.parametrize("seed", range(8))
+def test_random_uneven_batches_match_oracle(npde_mod, oracle, seed):
+    """Seeded batches of 12 random instances (the grids, boundaries and states of random_case) sharing one
+    scheme, limiter and launch shape: every instance equals its own oracle run -- step count, time and
+    bits -- or, when one fails, the batch reports the oracle's first non-physical cell of the first
+    failing instance."""
+    rng = np.random.default_rng(5000 + seed)
+    cases = [random_case(np.random.default_rng(int(rng.integers(1 << 30)))) for _ in range(12)]
+    _, _, scheme, lim, tuning = cases[0]
+    T = 0.03
+    expect = []
+    for g, U, _, _, _ in cases:
+        a = U.copy()
+        expect.append((oracle.run(g, scheme, lim, 0.5, a, T, nout=3), a))
+    with npde_mod.Batch([to_npde_grid(npde_mod, c[0]) for c in cases], scheme, lim, 0.5) as b:
+        b.set_tuning(**tuning)
+        for k, c in enumerate(cases):
+            b.set_state(k, c[1])
+        failing = [k for k, (ro, _) in enumerate(expect) if ro["bad"] is not None]
+        if not failing:
+            b.run_trajectory(T, 3, sizes=())
+            for k, (ro, a) in enumerate(expect):
+                t, steps = b.time(k)
+                assert steps == ro["steps"] and t == ro["t"], (seed, k, b.task_layout)
+                assert same(b.get_state(k), a), (seed, k, b.task_layout)
+        else:
+            with pytest.raises(npde_mod.NonPhysicalStateError) as e:
+                b.run_trajectory(T, 3, sizes=())
+            assert e.value.instance in failing, seed
+            assert (e.value.i, e.value.j) == expect[e.value.instance][0]["bad"], seed
+
+
 FAST_TOL_RANDOM = 1e-9  # max |fast - exact| / max|exact| per variable after the configuration's steps
 
 
