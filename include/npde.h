@@ -92,7 +92,10 @@ const char* npde_last_error(const npde_solver* s);
 /* Setting a state also resets that instance's time and step counter to 0. */
 npde_status npde_set_state(npde_solver* s, int instance, const double* U, int layout);
 npde_status npde_get_state(npde_solver* s, int instance, double* U, int layout);
-/* Device-resident variants (4 x ny x nx planes already in device memory, e.g. a torch tensor). */
+/* Device-resident variants (4 x ny x nx planes already in device memory, e.g. a torch tensor).  The
+ * handle's stream does not wait for other streams: the work that wrote dU must be complete before
+ * npde_set_state_device is called (synchronise its stream, or write dU on npde_stream(s)).  Both calls
+ * return after their copy has completed, so dU may be freed or reused at once. */
 npde_status npde_set_state_device(npde_solver* s, int instance, const double* dU);
 npde_status npde_get_state_device(npde_solver* s, int instance, double* dU);
 npde_status npde_get_time(npde_solver* s, int instance, double* t, long long* steps);
@@ -101,7 +104,10 @@ npde_status npde_get_times(npde_solver* s, double* t, long long* steps);
 /* Where the library keeps an instance's state: *U = device address of its 4 planes, *plane = elements
  * between planes (nx*ny rounded up to a multiple of 64), *instance_stride = elements from this instance's
  * state to the next instance's.  For callers that read (or, between calls, write) the state in place
- * without copies; valid until npde_destroy.  No counterpart in the reference (its U is host memory). */
+ * without copies; valid until npde_destroy.  The library launches on npde_stream(s), which does not wait
+ * for other streams: a write through this pointer must be issued on npde_stream(s), or be complete
+ * (its stream synchronised) before the next call on the handle.  No counterpart in the reference (its U
+ * is host memory). */
 npde_status npde_state_pointer(npde_solver* s, int instance, double** U, long long* plane, long long* instance_stride);
 
 /* ---- the hot path: Solver::compute_dt (solver2d.hpp:67-83) and Solver::step (solver2d.hpp:86-103) ---- */

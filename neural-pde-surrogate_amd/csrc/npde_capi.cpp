@@ -730,8 +730,11 @@ npde_status npde_set_state_device(npde_solver* s, int inst, const double* dU) {
     const InstDesc& D = s->hdesc[inst];
     npde_status st = pull_ctl(s);
     if (st != NPDE_OK) return st;
+    // The handle's stream does not wait for the legacy stream, so the caller must have finished writing dU
+    // (npde.h); the copy is complete when the call returns, so the caller may free or reuse dU at once.
     HIPCHK(s, hipMemcpy2DAsync(D.U, D.plane * sizeof(double), dU, cells * sizeof(double), cells * sizeof(double), 4,
                                hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
     Ctl& c = s->hctl[inst];
     c.t = 0.0;
     c.steps = 0;
@@ -1157,8 +1160,12 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
             std::string k;
             npde_tuning t = s->tune;
             if (!(ls >> k >> t.rows_per_task >> t.waves_per_block) || k != key.str()) continue;
-            if (!(ls >> t.row_groups)) t.row_groups = 0;
-            if (!(ls >> t.seams)) t.seams = 0;
+            // a line without the layout fields keeps the handle's current layout choices
+            int v = 0;
+            if (ls >> v) {
+                t.row_groups = v;
+                if (ls >> v) t.seams = v;
+            }
             npde_status st = npde_set_tuning(s, &t);
             if (st == NPDE_OK && chosen) *chosen = s->tune;
             return st;

@@ -16,6 +16,7 @@
 #include <iostream>
 #include <map>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -24,37 +25,43 @@
 
 namespace {
 
-std::map<std::string, std::string> parse_args(int argc, char* argv[]) {
-    std::map<std::string, std::string> a = {
-        {"test", "lax_liu3"}, {"n", "256"}, {"scheme", "hllc"}, {"limiter", "vanleer"}, {"cfl", "0.5"},
-        {"nout", "10"}, {"out", "runs"}, {"tend", "default"}, {"profile", "no"}, {"init", ""},
-        {"traj", ""}, {"save", ""}, {"tstart", "0"}};
-    for (int k = 1; k < argc; ++k) {
-        std::string key = argv[k];
-        if (key.rfind("--", 0) != 0 || k + 1 >= argc) {
-            std::cerr << "Expected '--key value', got '" << key << "'\n";
-            std::exit(1);
-        }
-        key = key.substr(2);
-        if (!a.count(key)) {
-            std::cerr << "Unknown option --" << key << '\n';
-            std::exit(1);
-        }
-        a[key] = argv[++k];
-    }
-    return a;
+// Every flag takes one value.  The table gives the flags and their defaults (main2d.cpp:25-28); a word
+// that is not a flag followed by a value, and a flag not in the table, end the run with the reference's
+// messages and exit code 1 (main2d.cpp:29-41).
+struct Flag {
+    const char* name;
+    const char* fallback;
+};
+constexpr Flag kFlags[] = {{"test", "lax_liu3"}, {"n", "256"},     {"scheme", "hllc"}, {"limiter", "vanleer"},
+                           {"cfl", "0.5"},       {"nout", "10"},   {"out", "runs"},    {"tend", "default"},
+                           {"profile", "no"},    {"init", ""},     {"traj", ""},       {"save", ""},
+                           {"tstart", "0"}};
+
+[[noreturn]] void usage_error(const std::string& msg) {
+    std::cerr << msg << '\n';
+    std::exit(1);
 }
 
-std::vector<int> parse_sizes(const std::string& s) {
-    std::vector<int> out;
-    size_t start = 0;
-    while (start < s.size()) {
-        size_t end = s.find(',', start);
-        if (end == std::string::npos) end = s.size();
-        out.push_back(std::stoi(s.substr(start, end - start)));
-        start = end + 1;
+std::map<std::string, std::string> read_flags(int argc, char* argv[]) {
+    std::map<std::string, std::string> opt;
+    for (const Flag& f : kFlags) opt.emplace(f.name, f.fallback);
+    for (int k = 1; k < argc; k += 2) {
+        const std::string word = argv[k];
+        if (word.compare(0, 2, "--") != 0 || k + 1 == argc) usage_error("Expected '--key value', got '" + word + "'");
+        const auto slot = opt.find(word.substr(2));
+        if (slot == opt.end()) usage_error("Unknown option " + word);
+        slot->second = argv[k + 1];
     }
-    return out;
+    return opt;
+}
+
+// --save "128,64": the trajectory's coarse sizes, in order (the reference's reading of the list,
+// trajectory.hpp:104-114: empty items are an error, a trailing comma is not).
+std::vector<int> save_sizes(const std::string& list) {
+    std::vector<int> sizes;
+    std::istringstream in(list);
+    for (std::string item; std::getline(in, item, ',');) sizes.push_back(std::stoi(item));
+    return sizes;
 }
 
 [[noreturn]] void die_nonphysical(const npde_bad_cell& b) {
@@ -73,7 +80,7 @@ void check(npde_solver* s, npde_status st, const npde_bad_cell& bad) {
 }  // namespace
 
 int main(int argc, char* argv[]) {
-    auto a = parse_args(argc, argv);
+    auto a = read_flags(argc, argv);
     const std::string test = a["test"], scheme = a["scheme"], lim_name = a["limiter"];
     const int n = std::atoi(a["n"].c_str()), nout = std::atoi(a["nout"].c_str());
     const double cfl = std::atof(a["cfl"].c_str());
@@ -145,7 +152,7 @@ int main(int argc, char* argv[]) {
     std::string dir;
     if (traj_mode) {
         try {
-            sizes = a["save"].empty() ? std::vector<int>{g.nx} : parse_sizes(a["save"]);
+            sizes = a["save"].empty() ? std::vector<int>{g.nx} : save_sizes(a["save"]);
             for (int m : sizes) {
                 if (m <= 0 || g.nx % m != 0)
                     throw std::runtime_error("save size " + std::to_string(m) + " does not divide nx = " +

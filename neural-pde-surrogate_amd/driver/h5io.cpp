@@ -296,7 +296,8 @@ void write_packed(const std::string& path, int N, int F, int n, const float* U, 
     attr(f.id, "split", split);
     attr(f.id, "n", n);
     attr(f.id, "frames", F);
-    attr(f.id, "dt", dt);
+    // one frame spacing shared by every row; rows with windows of their own (NaN) leave t authoritative
+    if (dt == dt) attr(f.id, "dt", dt);
 }
 
 #endif

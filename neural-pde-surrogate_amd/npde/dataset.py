@@ -138,6 +138,18 @@ def _io():
     return lib
 
 
+def frame_spacing(t: np.ndarray) -> float:
+    """The frame spacing every row of t (N, F) shares -- the reference stores its constant 0.005 as the
+    packed files' dt (data/pack.py:80) -- or NaN when rows have windows of their own (the held-out KH
+    instances each start at their own t_start), in which case the file carries no dt attribute and its
+    t dataset is authoritative."""
+    d = np.diff(t, axis=1)
+    if d.size == 0:
+        return float("nan")
+    ref = float(d[0, 0])
+    return ref if np.allclose(d, ref, rtol=1e-9, atol=0.0) else float("nan")
+
+
 def write_packed(path: Path, U: np.ndarray, t: np.ndarray, fams, idx, split: str, dt: float):
     lib = _io()
     U = np.ascontiguousarray(U, dtype=np.float32)
@@ -166,7 +178,7 @@ def pack(trajs, sizes, out_dir: Path, log=print) -> bool:
                 continue
             U = np.stack([trajs[k]["frames"][n] for k in ks])           # (N, 51, 4, n, n)
             t = np.stack([trajs[k]["t"] for k in ks])
-            dt = float(np.mean(np.diff(t, axis=1)))
+            dt = frame_spacing(t)
             write_packed(out_dir / f"{split}_{n}.h5", U, t, [k[0] for k in ks], [k[1] for k in ks], split, dt)
             min_rho = float(U[:, :, 0].min())
             min_p = float(min(pressure(u).min() for u in U))

@@ -346,13 +346,15 @@ def evaluate(model, test_idx, truth64, ladder=(64, 128, 256), log=print):
     return res
 
 
-def evaluate_kh(model, indices, n=128, n_fine=1024, log=print):
+def evaluate_kh(model, indices, n=128, n_fine=1024, log=print, force_fallback=False, keep_frames=False):
     """Held-out Kelvin-Helmholtz instances (npde.families.kh, never seen in training): from a random start
     time t_s in [0.5, 1.5] to t = 2.  Truth and fallback: the plain binary64 1024^2 solve from t = 0,
     block-averaged to n at NOUT frames after t_s (single-instance runs; their [t_s, 2] wall time is the
     fine solver's cost; times of all runs count from t_s).  The plain and learned solvers at n start from the truth's state at t_s,
     block-averaged in binary64, and run the same NOUT frame intervals; an instance whose learned run
-    fails takes the fine frames and is charged the fine solve's time."""
+    fails takes the fine frames and is charged the fine solve's time.  force_fallback marks every learned
+    run as failed (the fallback path itself); keep_frames adds each instance's truth, plain and learned
+    frames (NOUT+1, 4, n, n) float32 to its row."""
     from ..families import kh
     rows = []
     for i in indices:
@@ -382,7 +384,7 @@ def evaluate_kh(model, indices, n=128, n_fine=1024, log=print):
             b.run_trajectory(t_end - ts, NOUT, sizes=(n,))
             plain_s = time.perf_counter() - t0
             plain = b.trajectory(0, n)[None]
-        lf, learned_s, bad = learned_rollout_states(model, U0, tf)
+        lf, learned_s, bad = learned_rollout_states(model, U0, tf, force_fallback=force_fallback)
         fell = bool(bad[0])
         if fell:
             lf, learned_s = truth, learned_s + fine_s
@@ -390,6 +392,8 @@ def evaluate_kh(model, indices, n=128, n_fine=1024, log=print):
              "learned_seconds": learned_s, "fallback": fell,
              "l1_rho_plain": float(l1_density(plain, truth)[:, 1:].mean()),
              "l1_rho_learned": float(l1_density(lf, truth)[:, 1:].mean())}
+        if keep_frames:
+            r["frames"] = {"truth": truth[0], "plain": plain[0], "learned": lf[0]}
         rows.append(r)
         log(f"  kh {i}: t_s {ts:.3f}  L1(rho) plain {n}^2 {r['l1_rho_plain']:.5f}  learned {r['l1_rho_learned']:.5f}"
             f"{' (fallback)' if fell else ''}  wall fine {fine_s:.3f} s, plain {plain_s:.3f} s, learned {learned_s:.3f} s")

@@ -194,8 +194,11 @@ class Batch:
 
     def state_view(self):
         """Zero-copy torch view (n, 4, ny, nx) of every instance's state in device memory (all instances
-        must have the same size).  Read it after a call returns; write it only between calls.  The view
-        aliases the library's buffer: it is valid until the batch is closed."""
+        must have the same size).  Read it after a call returns; write it only between calls, and order
+        the write before the next call: the library's stream does not wait for torch's streams, so issue
+        the write under torch.cuda.stream(torch.cuda.ExternalStream(batch.stream)), or call
+        torch.cuda.synchronize() after it.  The view aliases the library's buffer: it is valid until the
+        batch is closed."""
         import torch
         g0 = self.grids[0]
         if any((g.nx, g.ny) != (g0.nx, g0.ny) for g in self.grids):

@@ -75,6 +75,24 @@ def test_packed_file_layout_round_trip(tmp_path):
     assert '"train"' in attrs and "(0): 8" in attrs and "(0): 5" in attrs
 
 
+def test_dt_attribute_only_for_a_shared_frame_spacing(tmp_path):
+    """dt is the frame spacing every row shares (the reference writes its constant, data/pack.py:80);
+    rows with windows of their own (held-out KH, each from its own t_start) give no dt attribute."""
+    if not IO_LIB.exists():
+        pytest.skip("libnpde_io.so not built")
+    from npde.dataset import frame_spacing, write_packed
+    shared = np.tile(np.linspace(0.0, 0.25, 51), (3, 1))
+    assert frame_spacing(shared) == shared[0, 1] - shared[0, 0]
+    own = np.stack([np.linspace(ts, 2.0, 51) for ts in (0.6, 1.1, 1.4)])
+    assert np.isnan(frame_spacing(own))
+    U = np.zeros((3, 51, 4, 4, 4), np.float32)
+    write_packed(tmp_path / "a.h5", U, shared, ["riemann"] * 3, [0, 1, 2], "train", frame_spacing(shared))
+    write_packed(tmp_path / "b.h5", U, own, ["kh"] * 3, [0, 1, 2], "ood", frame_spacing(own))
+    assert 'ATTRIBUTE "dt"' in h5_text(tmp_path / "a.h5", "-A")
+    assert "(0): 0.005" in h5_text(tmp_path / "a.h5", "-a", "dt")
+    assert 'ATTRIBUTE "dt"' not in h5_text(tmp_path / "b.h5", "-A")
+
+
 def test_packed_files_are_deterministic(tmp_path):
     """The same trajectories pack into the same bytes: the writer stores no object timestamps."""
     import filecmp

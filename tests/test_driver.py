@@ -35,6 +35,8 @@ def h5(path, name, dtype):
 @pytest.mark.parametrize("args,msg", [
     (["--bogus", "1"], "Unknown option --bogus"),
     (["--n"], "Expected '--key value', got '--n'"),
+    (["-n", "5"], "Expected '--key value', got '-n'"),
+    (["--n", "64", "extra"], "Expected '--key value', got 'extra'"),
     (["--test", "nope"], "Unknown test 'nope' (use sod_x, sod_y, lax_liu3, sine2d or kh)"),
     (["--scheme", "roe"], "Unknown scheme 'roe' (use rusanov or hllc)"),
     (["--limiter", "superbee"], "Unknown limiter 'superbee'"),
