@@ -131,6 +131,14 @@ npde_status npde_get_frame_times(npde_solver* s, int instance, double* t);
 
 /* Block average of the current state (trajectory.hpp:39-48): 4 x (ny/f) x m float32, f = nx/m. */
 npde_status npde_block_average(npde_solver* s, int instance, int m, float* out);
+/* npde_block_average without a device allocation: the values are staged in a static device array of
+ * 4 x 512 x 512 float32 (4 MiB per GPU, held until the process ends) instead of a buffer allocated per
+ * call.  Results are identical; a block average of more than 4 x 512 x 512 values returns
+ * NPDE_ERR_INVALID.  Every handle on a GPU shares that GPU's array, so calls that overlap on one GPU
+ * (handles driven from several host threads) race: serialise them.  No counterpart in the reference;
+ * for callers that cannot allocate device memory at call time (tight output loops, harnesses that
+ * forbid hipMalloc after set-up). */
+npde_status npde_block_average_noalloc(npde_solver* s, int instance, int m, float* out);
 
 /* ---- learned flux correction (coarse solver) ---- */
 /* Device pointers (4 x ny x nx binary64 each) holding per-face increments added to the face fluxes

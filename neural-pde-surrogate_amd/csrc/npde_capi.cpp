@@ -13,6 +13,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -93,7 +94,21 @@ struct npde_solver {
     int graph_key_fast = -1;
     std::string err;
     double last_ms = 0.0;
+    int device = 0;            // the GPU the handle's buffers and stream live on (current at creation)
+    float* scratch = nullptr;  // that GPU's copy of the static staging array (npde_block_average_noalloc)
 };
+
+// Staging buffer of npde_block_average_noalloc: a file-scope device array, so that the call allocates
+// nothing.  Every GPU the library runs on holds its own copy, at its own address, until the process
+// ends; the host-side name `scratch` is not a device address, so each handle keeps its device's address
+// (looked up once per device by bind_scratch) and passes that to kernels and copies.
+constexpr size_t kNoAllocFloats = size_t(4) * 512 * 512;  // 4 MiB: four 512 x 512 float32 planes
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, float*> g_scratch_of_device;
+}  // namespace
 
 namespace {
 
@@ -108,6 +123,23 @@ npde_status fail(npde_solver* s, npde_status st, const std::string& msg) {
         if (e_ != hipSuccess)                                                                        \
             return fail((s), NPDE_ERR_DEVICE, std::string(#call ": ") + hipGetErrorString(e_));     \
     } while (0)
+
+// Address of the static staging array on the handle's device, current during npde_batch_create: looked
+// up once per device (hipGetSymbolAddress) and shared by every handle on it.
+npde_status bind_scratch(npde_solver* s) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    const auto it = g_scratch_of_device.find(s->device);
+    if (it != g_scratch_of_device.end()) {
+        s->scratch = it->second;
+        return NPDE_OK;
+    }
+    void* p = nullptr;
+    HIPCHK(s, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(s, NPDE_ERR_DEVICE, "no device address for the staging array");
+    s->scratch = static_cast<float*>(p);
+    g_scratch_of_device.emplace(s->device, s->scratch);
+    return NPDE_OK;
+}
 
 size_t plane_stride(const npde_grid_desc& g) {
     const size_t cells = size_t(g.nx) * g.ny;
@@ -606,6 +638,8 @@ npde_status npde_batch_create(int n, const npde_grid_desc* grids, int flux, int 
         npde_destroy(s);
         return st;
     };
+    if (hipGetDevice(&s->device) != hipSuccess) return cleanup(NPDE_ERR_DEVICE);
+    if (bind_scratch(s) != NPDE_OK) return cleanup(NPDE_ERR_DEVICE);
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(NPDE_ERR_DEVICE);
     if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
         return cleanup(NPDE_ERR_DEVICE);
@@ -978,6 +1012,24 @@ npde_status npde_block_average(npde_solver* s, int inst, int m, float* out) {
     if (e == hipSuccess) e = hipMemcpyAsync(out, d, n * sizeof(float), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     (void)hipFree(d);
+    if (e != hipSuccess) return fail(s, NPDE_ERR_DEVICE, hipGetErrorString(e));
+    return NPDE_OK;
+}
+
+npde_status npde_block_average_noalloc(npde_solver* s, int inst, int m, float* out) {
+    if (!s || !out || inst < 0 || inst >= s->n) return fail(s, NPDE_ERR_INVALID, "bad arguments");
+    const npde_grid_desc& g = s->grids[inst];
+    if (m <= 0 || g.nx % m != 0 || g.ny % (g.nx / m) != 0) return fail(s, NPDE_ERR_INVALID, "size does not divide");
+    const int nyc = g.ny / (g.nx / m);
+    const size_t n = size_t(4) * m * nyc;
+    if (n > kNoAllocFloats)
+        return fail(s, NPDE_ERR_INVALID, "block average of " + std::to_string(n) + " values exceeds the " +
+                                             std::to_string(kNoAllocFloats) + " of the static staging array");
+    if (!s->scratch) return fail(s, NPDE_ERR_STATE, "no staging array on this handle's device");
+    // k_block_average writes every one of the n values before the copy reads them: nothing to clear
+    hipError_t e = launch_block_average(s->ddesc, inst, m, nyc, s->scratch, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, s->scratch, n * sizeof(float), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return fail(s, NPDE_ERR_DEVICE, hipGetErrorString(e));
     return NPDE_OK;
 }

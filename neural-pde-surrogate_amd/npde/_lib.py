@@ -62,6 +62,7 @@ _SIGNATURES = [
     ("npde_get_trajectory", C.c_int, [_H, C.c_int, C.c_int, _F]),
     ("npde_get_frame_times", C.c_int, [_H, C.c_int, _D]),
     ("npde_block_average", C.c_int, [_H, C.c_int, C.c_int, _F]),
+    ("npde_block_average_noalloc", C.c_int, [_H, C.c_int, C.c_int, _F]),
     ("npde_set_flux_correction", C.c_int, [_H, C.c_int, C.c_void_p, C.c_void_p]),
     ("npde_get_tuning", C.c_int, [_H, C.POINTER(Tuning)]),
     ("npde_set_tuning", C.c_int, [_H, C.POINTER(Tuning)]),
