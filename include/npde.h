@@ -141,13 +141,37 @@ npde_status npde_block_average(npde_solver* s, int instance, int m, float* out);
 npde_status npde_block_average_noalloc(npde_solver* s, int instance, int m, float* out);
 
 /* ---- learned flux correction (coarse solver) ---- */
-/* Device pointers (4 x ny x nx binary64 each) holding per-face increments added to the face fluxes
- * of every subsequent stage: corr_x[q][j][i] on the face left of cell (i, j), corr_y[q][j][i] on the
- * face below it (on a transmissive boundary the outermost faces get none).  Pass NULLs to switch the
+/* Device pointers (4 x ny x nx binary64 each) holding dimensionless per-face factors: every subsequent
+ * stage adds to the face flux of conserved component q the factor times the face's flux scale (rho a,
+ * rho a^2, rho a^2, rho a^3 for q = 0..3, with the smaller density and the larger sound speed of the two
+ * cells sharing the face, from that stage's input state).  corr_x[q][j][i] belongs to the face left of
+ * cell (i, j), corr_y[q][j][i] to the face below it (on a transmissive boundary the outermost faces get
+ * none).  Pass NULLs to switch the
  * correction off for that instance.  The pointers are read by the kernels until changed: the caller
  * keeps the buffers alive and finishes writing them before the next step.  nx*ny must be a multiple
  * of 64.  No counterpart in the reference. */
 npde_status npde_set_flux_correction(npde_solver* s, int instance, const double* corr_x, const double* corr_y);
+
+/* ---- learned flux-correction network run by the library (no counterpart in the reference) ----
+ * The fused inference kernel of the flux-correction network (8 features -> four 3x3 periodic convolutions
+ * of 32 channels with tanh-GELU -> a 3x3 head of 8 channels -> out_scale * tanh) writes dimensionless
+ * factors into correction planes the library allocates for every instance, in the layout and with the
+ * meaning of npde_set_flux_correction.  npde_run_trajectory evaluates it at every frame landing, for the
+ * instances that landed, from their state at that time (before their next step); npde_advance_to keeps
+ * the factors of the last evaluation.  weights: device pointer to the 69 packed bfloat16 MFMA fragments of
+ * the five layers (64 lanes x 8 values each: first layer 3 k-blocks x 2 n-tiles, three hidden layers 9 x 2,
+ * head 9 x 1; npde.learned.model.FluxCorrector.pack_fused writes them); biases: device pointer to
+ * 32 + 32 + 32 + 32 + 8 float32.  Both stay the caller's and must stay alive while the network is set.
+ * Every instance must be periodic with nx and ny multiples of 16.  weights = NULL switches it off. */
+npde_status npde_set_flux_network(npde_solver* s, const void* weights, const float* biases, double out_scale);
+/* Evaluate the network now for every instance (factors from the current states). */
+npde_status npde_flux_network_eval(npde_solver* s);
+/* The error indicator of the learned solver: per instance (host array of n), the largest density the network
+ * has seen at its evaluations since the last reset (0 if none); reset != 0 clears it after the read.  Non-
+ * finite and non-physical states are the solver's own check (NPDE_ERR_NONPHYSICAL). */
+npde_status npde_flux_network_peak(npde_solver* s, float* peak, int reset);
+/* Device addresses of an instance's correction planes written by the network (4 x ny x nx binary64 each). */
+npde_status npde_flux_network_planes(npde_solver* s, int instance, double** corr_x, double** corr_y);
 
 /* ---- built-in problems of the driver: sod_x, sod_y, lax_liu3, sine2d, kh (main2d.cpp:52-135) ---- */
 /* Fills grid and t_end; if U is non-NULL also the initial state as 4 x ny x nx planes.  Call once
@@ -168,6 +192,9 @@ int npde_conv3x3_wgrad(const float* x, const float* g, int B, int H, int W, int 
                        float* part, float* bpart, float* dw, float* db, void* stream);
 /* One 16x16x16 BF16 MFMA on row-major 16x16 float inputs (operand-layout check). */
 int npde_mfma_probe(const float* a, const float* b, float* d, void* stream);
+/* One 16x16x32 BF16 MFMA, D (16 x 16) = A (16 x 32) B (32 x 16), row-major floats (operand-layout check
+ * of the fused network kernel). */
+int npde_mfma32_probe(const float* a, const float* b, float* d, void* stream);
 
 /* ---- diagnostics (no counterpart in the reference) ---- */
 /* Element-wise on the device: q_fast = x / y by the kernels' shared-reciprocal sequence, q_ieee = x / y

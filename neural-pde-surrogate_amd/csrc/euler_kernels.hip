@@ -97,7 +97,8 @@ __device__ void finish_instance(const InstDesc& D, Ctl* c, const Partial* __rest
     }
 }
 
-enum FinishMode : int { kAfterStep = 0, kAfterWavespeed = 1, kAfterFrame = 2 };
+// kAfterFrameHold: as kAfterFrame, but the instance then holds (flux-network runs) until k_release
+enum FinishMode : int { kAfterStep = 0, kAfterWavespeed = 1, kAfterFrame = 2, kAfterFrameHold = 3 };
 
 // One wavefront per instance: after a stage-2 launch (instances that stepped), after a wave-speed
 // launch (instances not done), or after a frame launch (instances whose frame was just written).
@@ -109,18 +110,47 @@ __global__ __launch_bounds__(64) void k_finish(const InstDesc* __restrict__ desc
     Ctl* c = ctl + inst;
     const InstDesc& D = desc[inst];
     const int lane = threadIdx.x;
-    if (mode == kAfterFrame) {
+    if (mode == kAfterFrame || mode == kAfterFrameHold) {
         if (c->pending < 0 || lane != 0) return;
         const int k = c->pending;
         if (D.ftimes) D.ftimes[k] = c->t;
         c->pending = -1;
         c->k = k + 1;
+        // a flux-network run: the instance waits here for the batch's network call at this frame, unless
+        // that was its last frame
+        if (mode == kAfterFrameHold && c->k <= c->nout) {
+            c->hold = 1;
+            atomicAdd(ndone + kCountHeld, 1);
+            return;
+        }
         decide_next(D, c, ndone);
         return;
     }
     const bool fresh = mode == kAfterStep ? c->active != 0 : c->done == 0;
     if (!fresh) return;
     finish_instance(D, c, part, lane, ndone);
+}
+
+// Flux-network runs: once every instance that is not done holds at a frame landing (and the network has
+// run for them), release them all -- each decides its next step -- and clear the count.  One workgroup,
+// so the count is read before any instance moves and cleared after all have.
+__global__ __launch_bounds__(256) void k_release(const InstDesc* __restrict__ desc, Ctl* __restrict__ ctl, int n,
+                                                 int* __restrict__ counters) {
+    __shared__ int go;
+    if (threadIdx.x == 0) {
+        const int held = counters[kCountHeld];
+        go = held > 0 && held == n - counters[kCountDone];
+    }
+    __syncthreads();
+    if (!go) return;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        Ctl* c = ctl + k;
+        if (!c->hold) continue;
+        c->hold = 0;
+        decide_next(desc[k], c, counters);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) counters[kCountHeld] = 0;
 }
 
 // Max wave speeds and first non-physical cell of U for every instance that is not done.  Same task
@@ -256,7 +286,10 @@ __global__ void k_prep_host_step(Ctl* __restrict__ ctl, int n, const double* __r
 __global__ void k_begin_run(Ctl* __restrict__ ctl, int n, double t_start, double t_end, int nout, int record,
                             long long max_new_steps, int* __restrict__ ndone) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) *ndone = 0;
+    if (k == 0) {
+        ndone[kCountDone] = 0;
+        ndone[kCountHeld] = 0;
+    }
     if (k >= n) return;
     Ctl& c = ctl[k];
     c.step_limit = max_new_steps < 0 ? (long long)(~0ull >> 1) : c.steps + max_new_steps;
@@ -270,6 +303,7 @@ __global__ void k_begin_run(Ctl* __restrict__ ctl, int n, double t_start, double
     c.done = 0;
     c.error = 0;
     c.active = 0;
+    c.hold = 0;
 }
 
 // Arithmetic self-test, element-wise: the shared-reciprocal quotient and the plain IEEE quotient,
@@ -357,6 +391,11 @@ hipError_t launch_finish(const InstDesc* desc, Ctl* ctl, const Partial* part, in
 hipError_t launch_block_average(const InstDesc* desc, int inst, int m, int nyc, float* out, hipStream_t s) {
     const int cells = m * nyc;
     hipLaunchKernelGGL(k_block_average, dim3((cells + 255) / 256), dim3(256), 0, s, desc, inst, m, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_release(const InstDesc* desc, Ctl* ctl, int n, int* counters, hipStream_t s) {
+    hipLaunchKernelGGL(k_release, dim3(1), dim3(256), 0, s, desc, ctl, n, counters);
     return hipGetLastError();
 }
 

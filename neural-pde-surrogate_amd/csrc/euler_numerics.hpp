@@ -261,6 +261,27 @@ NPDE_HD Cons flux_x(const Side& A, const Side& B) {
     return {f.d, f.mn, f.mt, f.e};
 }
 
+// Flux scale of a face for the learned increments (no counterpart in the reference): r a, r a^2, r a^2,
+// r a^3 with a the larger sound speed of the two cells sharing the face and r = d^2 / (d + kCorrRho) for
+// d their smaller density (arguments: the cell left of / below the face, then the one right of / above
+// it).  The increment added to the face flux is the network's dimensionless factor times this scale,
+// evaluated on each stage's input state, so a correction stays a bounded fraction of the local flux as
+// the flow evolves.  r ~ d for d >> kCorrRho; below it the corrections fade out (r ~ d^2 / kCorrRho):
+// toward vacuum at finite pressure the sound speed, and with it the scale, grows without bound, and a
+// correction that removed a small fraction of a cell's mass per step could drain it step by step.
+constexpr double kCorrRho = 0.05;
+NPDE_HD Cons corr_scale(double dl, double al, double dr, double ar) {
+    const double d = tmin(dl, dr), a = tmax(al, ar);
+    const double r = ddiv(d * d, d + kCorrRho);
+    const double s1 = r * a, s2 = s1 * a;
+    return {s1, s2, s2, s2 * a};
+}
+
+// F + k * s, component-wise (a face flux plus its scaled increment)
+NPDE_HD Cons add_scaled(const Cons& F, const Cons& k, const Cons& s) {
+    return {F.d + k.d * s.d, F.m + k.m * s.m, F.n + k.n * s.n, F.e + k.e * s.e};
+}
+
 // Ghost index -> interior index it is copied from (grid2d.hpp:43-66); corners are never needed.
 NPDE_HD int remap(int i, int n, int bc) {
     if (i >= 0 && i < n) return i;

@@ -33,8 +33,13 @@ hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int
 hipError_t launch_block_average(const InstDesc* desc, int inst, int m, int nyc, float* out, hipStream_t s);
 hipError_t launch_finish(const InstDesc* desc, Ctl* ctl, const Partial* part, int n, int mode, int* ndone,
                          hipStream_t s);
-enum : int { kAfterStep = 0, kAfterWavespeed = 1, kAfterFrame = 2 };
+enum : int { kAfterStep = 0, kAfterWavespeed = 1, kAfterFrame = 2, kAfterFrameHold = 3 };
 hipError_t launch_prep_host_step(Ctl* ctl, int n, const double* dts, hipStream_t s);
+hipError_t launch_flux_net(const InstDesc* desc, const Ctl* ctl, const NetTask* tasks, int ntasks, int n,
+                           const int* counters, const void* frags, const float* bias, float out_scale, unsigned* peak,
+                           hipStream_t s);
+hipError_t launch_release(const InstDesc* desc, Ctl* ctl, int n, int* counters, hipStream_t s);
+hipError_t launch_mfma32_probe(const float* a, const float* b, float* d, hipStream_t s);
 hipError_t launch_begin_run(Ctl* ctl, int n, double t_start, double t_end, int nout, int record, long long max_new_steps,
                             int* ndone, hipStream_t s);
 hipError_t launch_selftest_arith(const double* x, const double* y, long long n, double* qf, double* qi, double* rt,
@@ -92,6 +97,18 @@ struct npde_solver {
     int graph_key_corr = -1;
     int graph_key_rows = -1, graph_key_wpb = -1;
     int graph_key_fast = -1;
+    int graph_key_net = -1;
+    // learned flux-correction network run at every frame landing (npde_set_flux_network): its weights and
+    // biases (the caller's device buffers), the bound of its factors, the correction planes it writes (8 per
+    // instance: face left of each cell, then face below it) and its tiles
+    const void* net_frags = nullptr;
+    const float* net_bias = nullptr;
+    float net_scale = 0.f;
+    double* dnetcorr = nullptr;
+    size_t netcorr_doubles = 0;
+    NetTask* dnettasks = nullptr;
+    int nnettasks = 0;
+    unsigned* dnetpeak = nullptr;  // per instance: the largest density seen at an evaluation (float bits)
     std::string err;
     double last_ms = 0.0;
     int device = 0;            // the GPU the handle's buffers and stream live on (current at creation)
@@ -362,7 +379,7 @@ long long steps_to_frame(const npde_solver* s) {
         if (!s->listed[k]) continue;
         const Ctl& c = s->hctl[k];
         if (c.done) continue;
-        if (c.pending >= 0) return 0;
+        if (c.pending >= 0 || c.hold) return 0;  // a frame (or the batch's network call) waits
         if (!c.record || c.nout <= 0) return -1;
         const double T = c.k == 0 ? c.t_start : c.t_start + (c.t_end - c.t_start) * double(c.k) / double(c.nout);
         const InstDesc& D = s->hdesc[k];
@@ -443,7 +460,17 @@ npde_status launch_iterations(npde_solver* s, int iters, int nft) {
     for (int it = 0; it < iters; ++it) {
         if (nft > 0) {
             HIPCHK(s, launch_frame(s->ddesc, s->dctl, s->dftasks, nft, s->dndone, s->stream));
-            HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterFrame, s->dndone, s->stream));
+            if (s->net_frags) {
+                // flux-network run: an instance that reached its frame holds there; once every running
+                // instance holds, the network turns their states into the next interval's factors (one
+                // launch for the batch) and they all move on
+                HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterFrameHold, s->dndone, s->stream));
+                HIPCHK(s, launch_flux_net(s->ddesc, s->dctl, s->dnettasks, s->nnettasks, s->n, s->dndone, s->net_frags,
+                                          s->net_bias, s->net_scale, s->dnetpeak, s->stream));
+                HIPCHK(s, launch_release(s->ddesc, s->dctl, s->n, s->dndone, s->stream));
+            } else {
+                HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterFrame, s->dndone, s->stream));
+            }
         }
         npde_status st = launch_step_pair(s, it == 0);
         if (st != NPDE_OK) return st;
@@ -456,7 +483,7 @@ npde_status ensure_graph(npde_solver* s, int nft, int level, bool frames) {
     const int K = std::max(1, s->tune.steps_per_graph);
     if (!(s->graph_key_frames == nft && s->graph_key_steps == K && s->graph_key_corr == int(s->has_corr) &&
           s->graph_key_rows == s->tune.rows_per_task && s->graph_key_wpb == s->tune.waves_per_block &&
-          s->graph_key_fast == int(s->fast))) {
+          s->graph_key_fast == int(s->fast) && s->graph_key_net == int(s->net_frags != nullptr))) {
         drop_graph(s);
         s->graph_key_frames = nft;
         s->graph_key_steps = K;
@@ -464,6 +491,7 @@ npde_status ensure_graph(npde_solver* s, int nft, int level, bool frames) {
         s->graph_key_rows = s->tune.rows_per_task;
         s->graph_key_wpb = s->tune.waves_per_block;
         s->graph_key_fast = int(s->fast);
+        s->graph_key_net = int(s->net_frags != nullptr);
     }
     hipGraphExec_t& slot = s->graphs[frames ? 1 : 0][level];
     if (slot) return NPDE_OK;
@@ -580,6 +608,7 @@ npde_status set_host_mode(npde_solver* s) {
         c.error = 0;
         c.active = 0;
         c.pending = -1;
+        c.hold = 0;
     }
     return push_ctl(s);
 }
@@ -646,7 +675,7 @@ npde_status npde_batch_create(int n, const npde_grid_desc* grids, int flux, int 
     if (hipMalloc(&s->dstate, total * sizeof(double)) != hipSuccess) return cleanup(NPDE_ERR_NOMEM);
     if (hipMemset(s->dstate, 0, total * sizeof(double)) != hipSuccess) return cleanup(NPDE_ERR_DEVICE);
     if (hipMalloc(&s->ddesc, sizeof(InstDesc) * n) != hipSuccess || hipMalloc(&s->dctl, sizeof(Ctl) * n) != hipSuccess ||
-        hipMalloc(&s->dndone, sizeof(int)) != hipSuccess || hipMalloc(&s->ddts, sizeof(double) * n) != hipSuccess)
+        hipMalloc(&s->dndone, sizeof(int) * kRunCounters) != hipSuccess || hipMalloc(&s->ddts, sizeof(double) * n) != hipSuccess)
         return cleanup(NPDE_ERR_NOMEM);
     for (int k = 0; k < n; ++k) {
         const npde_grid_desc& g = grids[k];
@@ -685,7 +714,8 @@ npde_status npde_destroy(npde_solver* s) {
     if (!s) return NPDE_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     drop_graph(s);
-    for (void* p : {(void*)s->ddesc, (void*)s->dctl, (void*)s->dtasks, (void*)s->dedge,
+    for (void* p : {(void*)s->ddesc, (void*)s->dctl, (void*)s->dtasks, (void*)s->dedge, (void*)s->dnetcorr,
+                    (void*)s->dnettasks, (void*)s->dnetpeak,
                     (void*)s->dftasks, (void*)s->dpart, (void*)s->dndone, (void*)s->ddts, (void*)s->dstate,
                     (void*)s->dtraj, (void*)s->dftimes})
         if (p) (void)hipFree(p);
@@ -730,6 +760,7 @@ npde_status npde_set_state(npde_solver* s, int inst, const double* U, int layout
     c.bad = kNoBad;
     c.error = 0;
     c.done = 0;
+    c.hold = 0;
     return push_ctl(s);
 }
 
@@ -776,6 +807,7 @@ npde_status npde_set_state_device(npde_solver* s, int inst, const double* dU) {
     c.bad = kNoBad;
     c.error = 0;
     c.done = 0;
+    c.hold = 0;
     return push_ctl(s);
 }
 
@@ -1052,6 +1084,92 @@ npde_status npde_set_flux_correction(npde_solver* s, int inst, const double* cor
     return NPDE_OK;
 }
 
+npde_status npde_set_flux_network(npde_solver* s, const void* weights, const float* biases, double out_scale) {
+    if (!s) return NPDE_ERR_INVALID;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    // off: the instances' corrections that pointed at the network's planes go, and with them the planes
+    if (!weights) {
+        for (auto& D : s->hdesc)
+            if (s->dnetcorr && D.corr_x >= s->dnetcorr && D.corr_x < s->dnetcorr + s->netcorr_doubles)
+                D.corr_x = D.corr_y = nullptr;
+        if (s->dnetcorr) (void)hipFree(s->dnetcorr);
+        if (s->dnettasks) (void)hipFree(s->dnettasks);
+        if (s->dnetpeak) (void)hipFree(s->dnetpeak);
+        s->dnetpeak = nullptr;
+        s->dnetcorr = nullptr;
+        s->netcorr_doubles = 0;
+        s->dnettasks = nullptr;
+        s->nnettasks = 0;
+        s->net_frags = nullptr;
+        s->net_bias = nullptr;
+        bool any = false;
+        for (const auto& d : s->hdesc) any = any || d.corr_x != nullptr;
+        s->has_corr = any;
+        drop_graph(s);
+        HIPCHK(s, hipMemcpy(s->ddesc, s->hdesc.data(), sizeof(InstDesc) * s->n, hipMemcpyHostToDevice));
+        return NPDE_OK;
+    }
+    if (!biases || !(out_scale >= 0.0)) return fail(s, NPDE_ERR_INVALID, "need biases and a non-negative bound");
+    for (const auto& g : s->grids)
+        if (g.bc_x != NPDE_BC_PERIODIC || g.bc_y != NPDE_BC_PERIODIC || g.nx % 16 || g.ny % 16)
+            return fail(s, NPDE_ERR_INVALID, "the flux network needs periodic instances with nx and ny multiples of 16");
+    if (!s->dnetcorr) {
+        size_t total = 0;
+        for (const auto& D : s->hdesc) total += size_t(8) * D.plane;
+        HIPCHK(s, hipMalloc(&s->dnetcorr, total * sizeof(double)));
+        s->netcorr_doubles = total;
+        HIPCHK(s, hipMemset(s->dnetcorr, 0, total * sizeof(double)));
+        std::vector<NetTask> tasks;
+        size_t off = 0;
+        for (int k = 0; k < s->n; ++k) {
+            InstDesc& D = s->hdesc[k];
+            D.corr_x = s->dnetcorr + off;
+            D.corr_y = s->dnetcorr + off + 4 * D.plane;
+            off += size_t(8) * D.plane;
+            for (int y0 = 0; y0 < D.ny; y0 += 16)
+                for (int x0 = 0; x0 < D.nx; x0 += 16) tasks.push_back({k, x0, y0, 0});
+        }
+        HIPCHK(s, hipMalloc(&s->dnetpeak, sizeof(unsigned) * s->n));
+        HIPCHK(s, hipMemset(s->dnetpeak, 0, sizeof(unsigned) * s->n));
+        HIPCHK(s, hipMalloc(&s->dnettasks, sizeof(NetTask) * tasks.size()));
+        HIPCHK(s, hipMemcpy(s->dnettasks, tasks.data(), sizeof(NetTask) * tasks.size(), hipMemcpyHostToDevice));
+        s->nnettasks = int(tasks.size());
+    }
+    s->net_frags = weights;
+    s->net_bias = biases;
+    s->net_scale = float(out_scale);
+    s->has_corr = true;
+    drop_graph(s);
+    HIPCHK(s, hipMemcpy(s->ddesc, s->hdesc.data(), sizeof(InstDesc) * s->n, hipMemcpyHostToDevice));
+    return NPDE_OK;
+}
+
+npde_status npde_flux_network_eval(npde_solver* s) {
+    if (!s) return NPDE_ERR_INVALID;
+    if (!s->net_frags) return fail(s, NPDE_ERR_STATE, "no flux network set");
+    HIPCHK(s, launch_flux_net(s->ddesc, s->dctl, s->dnettasks, s->nnettasks, s->n, nullptr, s->net_frags, s->net_bias,
+                              s->net_scale, s->dnetpeak, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return NPDE_OK;
+}
+
+npde_status npde_flux_network_peak(npde_solver* s, float* peak, int reset) {
+    if (!s) return NPDE_ERR_INVALID;
+    if (!s->dnetpeak) return fail(s, NPDE_ERR_STATE, "no flux network set");
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    if (peak) HIPCHK(s, hipMemcpy(peak, s->dnetpeak, sizeof(float) * s->n, hipMemcpyDeviceToHost));
+    if (reset) HIPCHK(s, hipMemset(s->dnetpeak, 0, sizeof(unsigned) * s->n));
+    return NPDE_OK;
+}
+
+npde_status npde_flux_network_planes(npde_solver* s, int inst, double** corr_x, double** corr_y) {
+    if (!s || inst < 0 || inst >= s->n || !corr_x || !corr_y) return fail(s, NPDE_ERR_INVALID, "bad arguments");
+    if (!s->dnetcorr) return fail(s, NPDE_ERR_STATE, "no flux network set");
+    *corr_x = const_cast<double*>(s->hdesc[inst].corr_x);
+    *corr_y = const_cast<double*>(s->hdesc[inst].corr_y);
+    return NPDE_OK;
+}
+
 npde_status npde_get_tuning(const npde_solver* s, npde_tuning* t) {
     if (!s || !t) return NPDE_ERR_INVALID;
     *t = s->tune;
@@ -1118,6 +1236,11 @@ npde_status npde_selftest_arith(const double* x, const double* y, long long n, d
     if (e == hipSuccess) e = hipMemcpy(root, d + 4 * n, b, hipMemcpyDeviceToHost);
     (void)hipFree(d);
     return e == hipSuccess ? NPDE_OK : NPDE_ERR_DEVICE;
+}
+
+int npde_mfma32_probe(const float* a, const float* b, float* d, void* stream) {
+    if (!a || !b || !d) return 1;
+    return launch_mfma32_probe(a, b, d, (hipStream_t)stream) == hipSuccess ? 0 : 3;
 }
 
 // Built-in initial conditions of the driver (main2d.cpp:52-135), evaluated with the host libm.

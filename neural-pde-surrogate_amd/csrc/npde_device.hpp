@@ -71,7 +71,13 @@ struct Ctl {
     int done;             // all targets reached (or failed)
     int error;            // 1: non-physical state met when a step was needed
     int bad;              // row-major index of the first non-physical cell, or kNoBad
+    int hold;             // flux-network runs: landed on a frame, waiting for the batch's network call
 };
+
+// Device counters of a run, one int each: instances done, and (flux-network runs) instances holding at
+// a frame landing.  The network runs, and the holding instances move on, once every instance that is not
+// done holds.
+enum RunCounter : int { kCountDone = 0, kCountHeld = 1, kRunCounters = 2 };
 
 struct Partial {
     double sx, sy;
@@ -84,6 +90,12 @@ struct Task {  // one wavefront of one RK stage
     int i0;  // first interior column of the strip
     int j0;  // first row of the band
     int j1;  // one past the last row of the band
+};
+
+struct NetTask {  // one workgroup of the flux-correction network: a 16 x 16 tile of one instance
+    int inst;
+    int x0, y0;  // first column and row of the tile
+    int pad;
 };
 
 struct FTask {  // one workgroup of the frame (block-average) kernel

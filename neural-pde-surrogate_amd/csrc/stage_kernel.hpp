@@ -200,12 +200,12 @@ __device__ __forceinline__ void seam_face(const InstDesc& D, const double* V, co
     const Side lf = edge_side(wr, slope<LIM>(wl, wr, w3), -0.5);
     F = flux_x<SCHEME>(rt, lf);
     if (corr && D.corr_x != nullptr && D.corr_y != nullptr) {
-        // the increment of the face left of column i0 (the right boundary face wraps when periodic)
+        // the increment of the face left of column i0 (the right boundary face wraps when periodic), scaled
+        // by the face's flux scale from the two cells beside it
         const int cxi = i0 < nx ? i0 : (D.bc_x == kPeriodic ? remap(i0, nx, kPeriodic) : -1);
-        if (cxi >= 0) {
-            const Cons k = load4(D.corr_x, r + cxi, P);
-            F = {F.d + k.d, F.m + k.m, F.n + k.n, F.e + k.e};
-        }
+        if (cxi >= 0)
+            F = add_scaled(F, load4(D.corr_x, r + cxi, P),
+                           corr_scale(wl.d, sound(wl.d, wl.p), wr.d, sound(wr.d, wr.p)));
     }
 }
 
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* 
     // row groups: edge values exchanged between the wavefronts of the workgroup.  s_w holds the edge
     // lanes' primitives of a row (double-buffered by row parity), s_rt lane 63's right-face side,
     // s_fl lane 0's left-face flux.
-    __shared__ double s_w[2][2][kMaxRowWaves][4];
+    __shared__ double s_w[2][2][kMaxRowWaves][5];  // primitives, then (flux corrections) the sound speed
     __shared__ double s_rt[kMaxRowWaves][9];
     __shared__ double s_fl[kMaxRowWaves][4];
     const int wl = ROWS ? (wv + nw - 1) % nw : 0;  // wavefront holding the columns to the left
@@ -319,20 +319,29 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* 
     const Prim w1 = prim_of(load4(V, row(j0 - 1), P));
     Prim wa = prim_of(va);
     Prim wb = prim_of(vb);
+    // flux corrections: the sound speeds of the carried rows, for the faces' flux scales
+    double aa = 0.0, ab = 0.0;
+    if constexpr (CORR) {
+        aa = sound(wa.d, wa.p);
+        ab = sound(wb.d, wb.p);
+    }
     Side top = edge_side(w1, slope<LIM>(w0, w1, wa), +0.5);
     Cons glo;
     {
         const Prim dy = slope<LIM>(w1, wa, wb);
         glo = flux_y<SCHEME>(top, edge_side(wa, dy, -0.5));
-        if constexpr (CORR) {
-            const Cons k = corr_y(j0);
-            glo = {glo.d + k.d, glo.m + k.m, glo.n + k.n, glo.e + k.e};
-        }
+        if constexpr (CORR) glo = add_scaled(glo, corr_y(j0), corr_scale(w1.d, sound(w1.d, w1.p), wa.d, aa));
         top = edge_side(wa, dy, +0.5);
     }
-    if constexpr (ROWS) {  // edge primitives of row j0 for the first x slopes
-        if (lane == 0) put(s_w[j0 & 1][0][wv], wa);
-        if (lane == kWave - 1) put(s_w[j0 & 1][1][wv], wa);
+    if constexpr (ROWS) {  // edge primitives (and sound speeds) of row j0 for the first x slopes
+        if (lane == 0) {
+            put(s_w[j0 & 1][0][wv], wa);
+            if constexpr (CORR) s_w[j0 & 1][0][wv][4] = aa;
+        }
+        if (lane == kWave - 1) {
+            put(s_w[j0 & 1][1][wv], wa);
+            if constexpr (CORR) s_w[j0 & 1][1][wv][4] = aa;
+        }
         __syncthreads();
     }
 
@@ -350,26 +359,33 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* 
         Cons uq{};
         if constexpr (STAGE == 2) uq = load4(O, (long long)(f + 1 < ny ? f + 1 : f) * nx + ci, P);
         const Prim wn = prim_of(vn);
+        double an = 0.0;
+        if constexpr (CORR) an = sound(wn.d, wn.p);
 
         // y: slope of row f+1, its two y-face sides, and the face between rows f and f+1
         const Prim dyb = slope<LIM>(wa, wb, wn);
         Cons ghi = flux_y<SCHEME>(top, edge_side(wb, dyb, -0.5));
-        if constexpr (CORR) {
-            const Cons k = corr_y(f + 1);
-            ghi = {ghi.d + k.d, ghi.m + k.m, ghi.n + k.n, ghi.e + k.e};
-        }
+        if constexpr (CORR) ghi = add_scaled(ghi, corr_y(f + 1), corr_scale(wa.d, aa, wb.d, ab));
         const Side topn = edge_side(wb, dyb, +0.5);
 
         // x: slope of this cell from the neighbouring lanes, left-face flux here, right-face flux from lane+1
-        Prim dxs;
+        Prim dxs, wp;  // wp: the cell to the left
+        double ap = 0.0;
         if constexpr (ROWS) {
-            dxs = slope<LIM>(from_prev_or(wa, s_w[f & 1][1][wl]), wa, from_next_or(wa, s_w[f & 1][0][wr]));
+            wp = from_prev_or(wa, s_w[f & 1][1][wl]);
+            if constexpr (CORR) ap = from_prev_or(aa, s_w[f & 1][1][wl][4]);
+            dxs = slope<LIM>(wp, wa, from_next_or(wa, s_w[f & 1][0][wr]));
         } else if constexpr (SEAMS) {
             const double* q = sseam + (f - j0) * kSeamLds + (lane >> 5) * (kSeamLds / 2);
             const double o[4] = {q[0], q[1], q[2], q[3]};
-            dxs = slope<LIM>(from_prev_or(wa, o), wa, from_next_or(wa, o));
+            wp = from_prev_or(wa, o);
+            // lane 0's left face is the seam, whose flux (increment included) comes from the seam solve
+            if constexpr (CORR) ap = from_prev(aa);
+            dxs = slope<LIM>(wp, wa, from_next_or(wa, o));
         } else {
-            dxs = slope<LIM>(from_prev(wa), wa, from_next(wa));
+            wp = from_prev(wa);
+            if constexpr (CORR) ap = from_prev(aa);
+            dxs = slope<LIM>(wp, wa, from_next(wa));
         }
         const Side lf = edge_side(wa, dxs, -0.5);
         const Side rt = edge_side(wa, dxs, +0.5);
@@ -382,10 +398,7 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* 
             rl = from_prev(rt);
         }
         Cons fl = flux_x<SCHEME>(rl, lf);
-        if constexpr (CORR) {
-            const Cons k = corr_x(f);
-            fl = {fl.d + k.d, fl.m + k.m, fl.n + k.n, fl.e + k.e};
-        }
+        if constexpr (CORR) fl = add_scaled(fl, corr_x(f), corr_scale(wp.d, ap, wa.d, aa));
         Cons fr;  // left-face flux of the column to the right
         if constexpr (ROWS) {
             // with it, the edge primitives of the next row (the slots of this row's parity were
@@ -393,8 +406,12 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* 
             if (lane == 0) {
                 put(s_fl[wv], fl);
                 put(s_w[(f + 1) & 1][0][wv], wb);
+                if constexpr (CORR) s_w[(f + 1) & 1][0][wv][4] = ab;
             }
-            if (lane == kWave - 1) put(s_w[(f + 1) & 1][1][wv], wb);
+            if (lane == kWave - 1) {
+                put(s_w[(f + 1) & 1][1][wv], wb);
+                if constexpr (CORR) s_w[(f + 1) & 1][1][wv][4] = ab;
+            }
             __syncthreads();
             fr = from_next_or(fl, s_fl[wr]);
         } else if constexpr (SEAMS) {
@@ -458,6 +475,10 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_stage(const InstDesc* 
         vb = vn;
         wa = wb;
         wb = wn;
+        if constexpr (CORR) {
+            aa = ab;
+            ab = an;
+        }
         vn = vq;
         if constexpr (STAGE == 2) ub = uq;
     };

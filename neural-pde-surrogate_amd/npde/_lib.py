@@ -79,6 +79,11 @@ _SIGNATURES = [
     ("npde_conv3x3_wgrad", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("npde_mfma_probe", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("npde_mfma32_probe", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("npde_set_flux_network", C.c_int, [_H, C.c_void_p, C.c_void_p, C.c_double]),
+    ("npde_flux_network_eval", C.c_int, [_H]),
+    ("npde_flux_network_peak", C.c_int, [_H, _F, C.c_int]),
+    ("npde_flux_network_planes", C.c_int, [_H, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
 ]
 
 EXPORTED = [name for name, _, _ in _SIGNATURES]

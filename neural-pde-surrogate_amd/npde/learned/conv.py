@@ -159,34 +159,49 @@ class FluxCNN(torch.autograd.Function):
 
 
 def main(argv=None):
-    """Time the corrector network's kernels at the training shape (forward + backward, batch 32 x 64^2)
-    and the rollout shape (forward, batch 64 x 64^2):  python -m npde.learned.conv [--iters 12]"""
+    """Time the corrector network at the training shape (layer kernels, forward + backward, batch 32 x 64^2)
+    and at the rollout shape (the fused inference kernel on a batch of 64 x 64^2 solver states, and the
+    layer kernels' forward for comparison):  python -m npde.learned.conv [--iters 20]"""
     import argparse
     import time
-    from .model import FluxCorrector
+    from .. import solver as S
+    from ..families import riemann
+    from .model import FluxCorrector, features
     ap = argparse.ArgumentParser()
-    ap.add_argument("--iters", type=int, default=12)
+    ap.add_argument("--iters", type=int, default=20)
     args = ap.parse_args(argv)
     dev = torch.device("cuda")
     m = FluxCorrector().to(dev)
     with torch.no_grad():
         m.weights[-1].normal_(0.0, 0.1)
     g = torch.Generator(device=dev).manual_seed(0)
-    for B, training in ((32, True), (64, False)):
-        x = torch.randn(B, 64, 64, 16, device=dev, generator=g)
-        x[..., 8:] = 0
-        packed = None if training else pack_stack(m.params())
+
+    def timed(f):
+        f()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.iters):
-            if training:
-                FluxCNN.apply(x, *m.params()).sum().backward()
-            else:
-                with torch.no_grad():
-                    flux_cnn_infer(x, packed)
+            f()
         torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) * 1e3 / args.iters
-        print(f"batch {B} x 64^2 {'forward + backward' if training else 'forward'}: {ms:.3f} ms per call", flush=True)
+        return (time.perf_counter() - t0) * 1e3 / args.iters
+
+    x = torch.randn(32, 64, 64, 16, device=dev, generator=g)
+    x[..., 8:] = 0
+    ms = timed(lambda: FluxCNN.apply(x, *m.params()).sum().backward())
+    print(f"layer kernels, batch 32 x 64^2, forward + backward: {ms:.3f} ms per call", flush=True)
+    B, n = 64, 64
+    with S.Batch([S.make_grid(n, n, bc_x=1, bc_y=1)] * B, "hllc", "vanleer", 0.5) as b:
+        for k in range(B):
+            b.set_state(k, riemann(k, n)[0])
+        b.set_flux_network(m)
+        ms = timed(b.flux_network_eval)
+        print(f"fused kernel, batch {B} x {n}^2 from the solver states: {ms:.3f} ms per call "
+              f"(host-synchronised; the device time is in the kernel trace)", flush=True)
+        U = b.state_view()
+        packed = pack_stack(m.params())
+        with torch.no_grad():
+            ms = timed(lambda: flux_cnn_infer(features(U), packed))
+        print(f"layer kernels, batch {B} x {n}^2 forward with features: {ms:.3f} ms per call", flush=True)
 
 
 if __name__ == "__main__":

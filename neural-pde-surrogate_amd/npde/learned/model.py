@@ -1,19 +1,21 @@
 """Conservative learned flux correction for the coarse solver.
 
 A stack of 3x3 periodic convolutions (4 hidden layers of 32 channels with tanh-GELU, then a linear
-head; 32,392 parameters) maps the coarse state to 8 per-face flux increments: 4 conserved components
-on the face left of each cell and 4 on the face below it.  Each increment is out_scale * tanh(raw)
-times the face's natural flux scale (rho a, rho a^2, rho a^2, rho a^3 with the smaller density and
-the larger sound speed of the two cells sharing the face), so a correction stays a bounded fraction of
-the flux it corrects, also in rarefied, low-pressure regions where a fixed-size increment would drive
-the state non-physical.  The increments are added to the solver's
-face fluxes (HLLC + van Leer) inside the stage kernels, so the update stays in flux form: mass,
-momentum and energy are conserved to rounding whatever the network outputs.  The head is zero
-initialised, so an untrained corrector reproduces the plain solver exactly (analog: the zero-init
+head; 32,392 parameters) maps the coarse state to 8 per-face factors: 4 conserved components on the
+face left of each cell and 4 on the face below it, each out_scale * tanh(raw).  The solver adds to each
+face flux its factor times the face's natural flux scale (rho a, rho a^2, rho a^2, rho a^3 with the
+smaller density and the larger sound speed of the two cells sharing the face), evaluated on every RK
+stage's input state (csrc/euler_numerics.hpp corr_scale; npde.learned.torch_solver.face_scale): a
+correction stays a bounded fraction of the local flux as the flow evolves between network calls, also
+in rarefied, low-pressure regions where a fixed-size increment would drive the state non-physical.  The
+increments enter the face fluxes (HLLC + van Leer) inside the stage kernels, so the update stays in flux
+form: mass, momentum and energy are conserved to rounding whatever the network outputs.  The head is
+zero initialised, so an untrained corrector reproduces the plain solver exactly (analog: the zero-init
 last layer of the reference's surrogates, surrogate/models.py:66-69).
 
 Inputs are 8 features per cell (conserved and primitive variables and sound speed, shifted and
-scaled to order one) padded to 16 channels for the MFMA kernels.
+scaled to order one); the per-layer training kernels take them padded to 16 channels, the fused
+inference kernel (npde_flux_net) takes the 8 and computes them itself from the state.
 """
 from __future__ import annotations
 
@@ -21,7 +23,7 @@ import math
 
 import torch
 
-from .conv import FluxCNN, flux_cnn_infer, pack_stack
+from .conv import FluxCNN
 
 GAMMA = 1.4
 IN_FEATURES = 8
@@ -44,17 +46,6 @@ def features(U: torch.Tensor) -> torch.Tensor:
     vals = [d, U[:, 1], U[:, 2], U[:, 3], u, v, p, a]
     f = torch.stack([(x - o) / s for x, (o, s) in zip(vals, FEATURE_NORM)], dim=-1).to(torch.float32)
     return torch.cat([f, torch.zeros_like(f)], dim=-1).contiguous()
-
-
-def face_scales(U: torch.Tensor):
-    """Flux scales (B, 4, H, W) of the face left of each cell (x) and below it (y), periodic neighbours."""
-    d, _, _, _, a = primitives(U)
-    out = []
-    for dim in (-1, -2):
-        df = torch.minimum(d, d.roll(1, dims=dim))
-        af = torch.maximum(a, a.roll(1, dims=dim))
-        out.append(torch.stack([df * af, df * af * af, df * af * af, df * af * af * af], dim=1))
-    return out
 
 
 class FluxCorrector(torch.nn.Module):
@@ -83,28 +74,44 @@ class FluxCorrector(torch.nn.Module):
         return sum(p.numel() for p in self.parameters())
 
     def forward(self, U: torch.Tensor):
-        """(B, 4, H, W) float64 state -> (corr_x, corr_y), each (B, 4, H, W) float64."""
+        """(B, 4, H, W) float64 state -> (corr_x, corr_y), each (B, 4, H, W) float64 dimensionless factors
+        of the face left of / below each cell (the solver multiplies them by the face flux scales)."""
         # the network's input is not differentiated: gradients reach it through the flux increments only
-        Ud = U.detach()
-        out = FluxCNN.apply(features(Ud), *self.params())  # (B, H, W, 8) float32
+        out = FluxCNN.apply(features(U.detach()), *self.params())  # (B, H, W, 8) float32
         out = (torch.tanh(out) * self.out_scale).to(torch.float64)
-        sx, sy = face_scales(Ud)
-        cx = (out[..., 0:4].permute(0, 3, 1, 2) * sx).contiguous()
-        cy = (out[..., 4:8].permute(0, 3, 1, 2) * sy).contiguous()
-        return cx, cy
+        return out[..., 0:4].permute(0, 3, 1, 2).contiguous(), out[..., 4:8].permute(0, 3, 1, 2).contiguous()
 
-    def increments_into(self, U: torch.Tensor, CX: torch.Tensor, CY: torch.Tensor, keep: torch.Tensor, packed):
-        """Inference: the increments of forward(U) written into CX, CY (B, 4, H, W) float64, zero for
-        instances whose keep flag is False.  `packed` = pack_stack(self.params()), computed once; no
-        tensor is allocated outside torch's allocator and nothing synchronises the host, so the call
-        can be captured in a graph."""
-        out = flux_cnn_infer(features(U), packed)
-        out = (torch.tanh(out) * self.out_scale).to(torch.float64)
-        sx, sy = face_scales(U)
-        k = keep.view(-1, 1, 1, 1)
-        zero = torch.zeros((), dtype=torch.float64, device=U.device)
-        CX.copy_(torch.where(k, out[..., 0:4].permute(0, 3, 1, 2) * sx, zero))
-        CY.copy_(torch.where(k, out[..., 4:8].permute(0, 3, 1, 2) * sy, zero))
+    def pack_fused(self):
+        """The weights and biases in the operand order of the fused inference kernel (npde_set_flux_network):
+        (fragments, biases, out_scale) with fragments an int16 tensor of 69 x 64 x 8 bfloat16 bit patterns
+        (round to nearest even) and biases 136 float32, on the model's device.  A fragment holds, for each of
+        the 64 lanes l (m = l % 16, g = l / 16), the 8 k-values of output channel 16 t + m: input channels
+        8 g .. 8 g + 7 of tap b for the 32-channel layers (k-block b = tap), the 8 features of tap 4 b + g for
+        the first layer (zero past the ninth tap); tap = ky * 3 + kx of the 3 x 3 cross-correlation."""
+        def first(w):  # (32, 8, 3, 3) -> (3, 2, 4, 16, 8)
+            wt = w.reshape(32, 8, 9)
+            out = w.new_zeros(3, 2, 4, 16, 8)
+            for b in range(3):
+                for g in range(4):
+                    if 4 * b + g < 9:
+                        out[b, :, g] = wt[:, :, 4 * b + g].reshape(2, 16, 8)
+            return out
+
+        def hidden(w, cout_p):  # (cout, 32, 3, 3) -> (9, cout_p / 16, 4, 16, 8)
+            cout = w.shape[0]
+            wt = w.new_zeros(cout_p, 32, 9)
+            wt[:cout] = w.reshape(cout, 32, 9)
+            return wt.permute(2, 0, 1).reshape(9, cout_p // 16, 16, 4, 8).permute(0, 1, 3, 2, 4)
+
+        ws = [w.detach().float() for w in self.weights]
+        if [tuple(w.shape) for w in ws] != [(32, 8, 3, 3)] + [(32, 32, 3, 3)] * 3 + [(8, 32, 3, 3)]:
+            raise ValueError("pack_fused needs the 8 -> 4 x 32 -> 8 stack")
+        frags = torch.cat([first(ws[0]).reshape(-1)] + [hidden(w, 32).reshape(-1) for w in ws[1:4]]
+                          + [hidden(ws[4], 16).reshape(-1)])
+        assert frags.numel() == 69 * 64 * 8
+        bits = frags.to(torch.bfloat16).view(torch.int16).contiguous()
+        biases = torch.cat([b.detach().float() for b in self.biases]).contiguous()
+        return bits, biases, float(self.out_scale)
 
     def state(self) -> dict:
         return {f"w{k}": w.detach().cpu() for k, w in enumerate(self.weights)} | \

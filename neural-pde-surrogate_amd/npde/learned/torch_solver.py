@@ -10,8 +10,10 @@ vectorised over the four variables and over both face directions (x and y faces 
 Riemann call), which keeps the number of launched ops per step small.  Unselected branches use
 harmless denominators so their gradients stay finite; the selected values are unchanged.
 
-State: (B, 4, ny, nx) float64 on a periodic grid.  Corrections: (B, 4, ny, nx) each, cx on the face
-left of a cell and cy on the face below it, added to the face fluxes in both RK stages.
+State: (B, 4, ny, nx) float64 on a periodic grid.  Corrections: (B, 4, ny, nx) each, dimensionless
+factors for the face left of a cell (cx) and the face below it (cy); in both RK stages each face flux
+gets its factor times the face's flux scale (face_scale, from the stage's input state), as the HIP stage
+kernels do (csrc/euler_numerics.hpp corr_scale).
 """
 from __future__ import annotations
 
@@ -123,6 +125,20 @@ def rusanov(A, B):
 _YSWAP = {"d": "d", "mn": "mt", "mt": "mn", "e": "e", "un": "ut", "ut": "un", "p": "p", "a": "a"}
 
 
+CORR_RHO = 0.05  # densities below this fade the corrections out (csrc/euler_numerics.hpp kCorrRho)
+
+
+def face_scale(dl, al, dr, ar):
+    """(B, H, W) densities and sound speeds of the cells left of / below (l) and right of / above (r) each
+    face -> (B, 4, H, W) flux scales r a, r a^2, r a^2, r a^3 with a the larger sound speed and
+    r = d^2 / (d + CORR_RHO) for d the smaller density (the kernels' corr_scale, same operation order)."""
+    d, a = _tmin(dl, dr), _tmax(al, ar)
+    r = (d * d) / (d + CORR_RHO)
+    s1 = r * a
+    s2 = s1 * a
+    return torch.stack([s1, s2, s2, s2 * a], dim=1)
+
+
 def rhs(U, dx, dy, scheme="hllc", lim="vanleer", cx_corr=None, cy_corr=None):
     """L(U) for a periodic batch (B, 4, ny, nx); x faces first, then y (solver2d.hpp:129-180)."""
     riem = hllc if scheme == "hllc" else rusanov
@@ -144,10 +160,12 @@ def rhs(U, dx, dy, scheme="hllc", lim="vanleer", cx_corr=None, cy_corr=None):
     f = riem(A, Bs)                                                        # normal frame, (dir, B, H, W)
     Fl = torch.stack([f[0][0], f[1][0], f[2][0], f[3][0]], dim=1)          # x faces: (rho, mx, my, E)
     Gb = torch.stack([f[0][1], f[2][1], f[1][1], f[3][1]], dim=1)          # y faces back to (rho, mx, my, E)
-    if cx_corr is not None:
-        Fl = Fl + cx_corr
-    if cy_corr is not None:
-        Gb = Gb + cy_corr
+    if cx_corr is not None or cy_corr is not None:
+        a = _sqrt((GAMMA * p) / d)
+        if cx_corr is not None:
+            Fl = Fl + cx_corr * face_scale(torch.roll(d, 1, -1), torch.roll(a, 1, -1), d, a)
+        if cy_corr is not None:
+            Gb = Gb + cy_corr * face_scale(torch.roll(d, 1, -2), torch.roll(a, 1, -2), d, a)
     cx, cy = 1.0 / dx, 1.0 / dy
     Lx = (0.0 + cx * Fl) - cx * torch.roll(Fl, -1, -1)
     return (Lx + cy * Gb) - cy * torch.roll(Gb, -1, -2)

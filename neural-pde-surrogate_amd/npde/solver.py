@@ -265,9 +265,10 @@ class Batch:
         sizes = [int(m) for m in sizes]
         arr = (C.c_int * max(1, len(sizes)))(*sizes)
         bad = BadCell()
+        # set first: when an instance fails, the others' frames are complete and can still be read
+        self._traj = (sizes, int(nout))
         check(self._h, self._lib.npde_run_trajectory(self._h, float(t_start), float(t_end), int(nout), arr,
                                                      len(sizes), C.byref(bad)), bad)
-        self._traj = (sizes, int(nout))
 
     def trajectory(self, inst: int, m: int) -> np.ndarray:
         sizes, nout = self._traj
@@ -296,6 +297,37 @@ class Batch:
         check(self._h, self._lib.npde_set_flux_correction(
             self._h, inst, C.c_void_p(corr_x_ptr) if corr_x_ptr else None,
             C.c_void_p(corr_y_ptr) if corr_y_ptr else None))
+
+    def set_flux_network(self, model=None):
+        """Run the learned flux-correction network (a FluxCorrector of the 8 -> 4 x 32 -> 8 stack) inside
+        the device loop: npde_run_trajectory evaluates it at every frame landing of every instance, from the
+        state at that time, and the stage kernels add its scaled factors to the face fluxes until the next
+        landing.  None switches it off.  The packed weights stay referenced by the batch."""
+        if model is None:
+            check(self._h, self._lib.npde_set_flux_network(self._h, None, None, 0.0))
+            self._net = None
+            return
+        frags, biases, scale = model.pack_fused()
+        self._net = (frags, biases)  # device buffers the library reads until switched off
+        check(self._h, self._lib.npde_set_flux_network(self._h, C.c_void_p(frags.data_ptr()),
+                                                       C.c_void_p(biases.data_ptr()), scale))
+
+    def flux_network_eval(self):
+        """Evaluate the network now on every instance's current state (its factors for the next steps)."""
+        check(self._h, self._lib.npde_flux_network_eval(self._h))
+
+    def flux_network_peak(self, reset: bool = False) -> np.ndarray:
+        """Per instance, the largest density the network has seen at its evaluations since the last reset."""
+        out = np.zeros(self.n, dtype=np.float32)
+        check(self._h, self._lib.npde_flux_network_peak(self._h, out.ctypes.data_as(C.POINTER(C.c_float)),
+                                                        int(bool(reset))))
+        return out
+
+    def flux_network_planes(self, inst: int):
+        """(corr_x, corr_y) device addresses of the network's factor planes of an instance."""
+        x, y = C.c_void_p(), C.c_void_p()
+        check(self._h, self._lib.npde_flux_network_planes(self._h, inst, C.byref(x), C.byref(y)))
+        return x.value, y.value
 
     # -- performance --
     @property

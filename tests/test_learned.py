@@ -81,6 +81,55 @@ def test_corrector_shape_and_identity_at_init():
     assert all(float(w.abs().max()) == 0.0 for w in [m.weights[-1]])
 
 
+def test_dihedral_maps_commute_with_the_step():
+    """The training augmentation's 8 symmetries of the periodic square are symmetries of the scheme: one
+    SSP-RK2 step of a mapped state equals the mapped step to rounding (a wrong momentum sign or exchange
+    fails by O(1)), and the maps form the group (each has an inverse in it)."""
+    from npde.learned import torch_solver as ts
+    from npde.learned.pipeline import dihedral
+    n = 24
+    U0 = torch.from_numpy(riemann_batch(n, [7])).expand(8, 4, n, n).contiguous()
+    g = torch.arange(8)
+    dt = torch.full((8,), 0.002, dtype=torch.float64)
+    a = ts.step(dihedral(U0, g), dt, 1 / n, 1 / n)
+    b = dihedral(ts.step(U0, dt, 1 / n, 1 / n), g)
+    assert float((a - b).abs().max()) < 1e-12 * float(U0.abs().max())
+    assert float((a - ts.step(U0, dt, 1 / n, 1 / n)).abs().max()) > 1e-3  # the maps do change the state
+    inv = torch.tensor([0, 1, 2, 3, 4, 6, 5, 7])  # transpose after a single mirror: inverse mirrors the other axis
+    assert torch.equal(dihedral(dihedral(U0, g), inv), U0)
+
+
+def test_fused_packing_places_every_weight():
+    """pack_fused: fragment (layer, k-block b, n-tile t), lane l = 16 g + m, value j holds W[16 t + m][ci][tap]
+    with (tap, ci) = (b, 8 g + j) for the 32-channel layers and (4 b + g, j) for the first layer; zero
+    where the head has no channel or the first layer no tap; biases in layer order."""
+    from npde.learned.model import FluxCorrector
+    m = FluxCorrector(seed=3)
+    g = torch.Generator().manual_seed(3)
+    with torch.no_grad():
+        for w in m.weights:
+            w.copy_(torch.randn(w.shape, generator=g))
+        for b in m.biases:
+            b.copy_(torch.randn(b.shape, generator=g))
+    bits, biases, scale = m.pack_fused()
+    fr = bits.view(torch.bfloat16).float().reshape(69, 64, 8)
+    ws = [w.detach().to(torch.bfloat16).float() for w in m.weights]
+    starts, kbs, nts = [0, 6, 24, 42, 60], [3, 9, 9, 9, 9], [2, 2, 2, 2, 1]
+    for L in range(5):
+        W = ws[L]
+        for b in range(kbs[L]):
+            for t in range(nts[L]):
+                for lane in range(64):
+                    mm, gg = lane % 16, lane // 16
+                    co = 16 * t + mm
+                    for j in range(8):
+                        tap, ci = (4 * b + gg, j) if L == 0 else (b, 8 * gg + j)
+                        want = 0.0 if (tap >= 9 or co >= W.shape[0]) else float(W[co, ci, tap // 3, tap % 3])
+                        assert float(fr[starts[L] + b * nts[L] + t, lane, j]) == want, (L, b, t, lane, j)
+    assert torch.equal(biases, torch.cat([b.detach() for b in m.biases]))
+    assert scale == m.out_scale
+
+
 # ------------------------------------------------------------------------------------------- GPU
 @pytest.fixture(scope="module")
 def cuda(npde_mod):
@@ -100,6 +149,22 @@ def test_mfma_operand_layout(cuda):
     assert rc == 0
     torch.cuda.synchronize()
     assert torch.equal(d, a @ b)  # small integers: exact in bf16 and in fp32 accumulation
+
+
+@pytest.mark.gpu
+def test_mfma32_operand_layout(cuda):
+    """The 16x16x32 BF16 MFMA operand layout the fused network kernel assumes (small integers: exact)."""
+    import ctypes as C
+    import npde._lib as L
+    lib = L.load()
+    a = torch.arange(512, dtype=torch.float32, device=cuda).reshape(16, 32) % 7 - 3
+    b = torch.arange(512, dtype=torch.float32, device=cuda).reshape(32, 16) % 5 - 2
+    d = torch.empty(16, 16, dtype=torch.float32, device=cuda)
+    rc = lib.npde_mfma32_probe(C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), C.c_void_p(d.data_ptr()),
+                               C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert torch.equal(d, a @ b)
 
 
 def _random_net(seed, cuda, head_scale=0.3):
@@ -272,3 +337,80 @@ def test_held_out_kh_evaluation_runs_and_charges_fallbacks(npde_mod, cuda):
             assert r["l1_rho_learned"] == 0.0 and r["learned_seconds"] >= r["fine_seconds"]
         else:
             assert np.isfinite(r["l1_rho_learned"]) and r["l1_rho_learned"] > 0.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [16, 64, 96])
+def test_fused_network_kernel_matches_oracle(npde_mod, cuda, n):
+    """The fused inference kernel (features, four GELU layers and the head in one kernel, activations in
+    LDS as bfloat16, 16x16x32 MFMAs) on solver states against the float64 network oracle with the same
+    bfloat16 roundings, on grids of one, four and six tiles per side: the factors out_scale * tanh(head)
+    agree to 2e-3 of their largest magnitude, and the instances of a batch are independent."""
+    import nn_oracle
+    from npde.learned.model import features
+    m = _random_net(5, cuda, head_scale=0.05)
+    m.out_scale = 1.0
+    B = 3
+    U0 = riemann_batch(n, range(10, 10 + B))
+    with npde_mod.Batch([npde_mod.make_grid(n, n, bc_x=1, bc_y=1)] * B, "hllc", "vanleer", 0.5) as b:
+        for k in range(B):
+            b.set_state(k, U0[k])
+        b.set_flux_network(m)
+        b.flux_network_eval()
+        got = []
+        for k in range(B):  # the call returned after the kernel completed: read the planes in place
+            px, py = b.flux_network_planes(k)
+            cx, cy = (torch.as_tensor(_DevView(p, (4, n, n)), device=cuda).clone() for p in (px, py))
+            got.append(torch.cat([cx, cy]).permute(1, 2, 0).cpu())
+        peak = b.flux_network_peak()
+    x = features(torch.from_numpy(U0)).double()
+    ws = [w.detach().double().cpu() for w in m.weights]
+    bs = [bb.detach().double().cpu() for bb in m.biases]
+    want = torch.tanh(nn_oracle.forward(x, ws, bs)) * m.out_scale  # (B, n, n, 8)
+    for k in range(B):
+        err = float((got[k] - want[k]).abs().max() / want[k].abs().max())
+        assert err < 2e-3, (n, k, err)
+    assert np.array_equal(peak, U0[:, 0].reshape(B, -1).max(axis=1).astype(np.float32))
+
+
+class _DevView:
+    """A (shape) float64 view of a device address, for torch.as_tensor."""
+
+    def __init__(self, ptr, shape):
+        strides = []
+        acc = 8
+        for d in reversed(shape):
+            strides.insert(0, acc)
+            acc *= d
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": "<f8", "data": (ptr, False),
+                                         "version": 3, "strides": tuple(strides)}
+
+
+@pytest.mark.gpu
+def test_network_in_the_device_loop_equals_a_host_loop(npde_mod, cuda):
+    """npde_run_trajectory with the network set evaluates it at every frame landing of every instance: the
+    frames and final states equal a host-driven loop that evaluates the network on every instance and then
+    advances to the next frame time, bit for bit (the instances land at different step counts)."""
+    m = _random_net(6, cuda, head_scale=0.3)
+    m.out_scale = 0.1
+    n, B, T, NOUT = 64, 4, 0.06, 6
+    U0 = riemann_batch(n, range(20, 20 + B))
+    grids = [npde_mod.make_grid(n, n, bc_x=1, bc_y=1)] * B
+    with npde_mod.Batch(grids, "hllc", "vanleer", 0.5) as a, npde_mod.Batch(grids, "hllc", "vanleer", 0.5) as h:
+        for k in range(B):
+            a.set_state(k, U0[k])
+            h.set_state(k, U0[k])
+        a.set_flux_network(m)
+        h.set_flux_network(m)
+        a.run_trajectory(T, NOUT, sizes=(n,))
+        frames = {k: a.trajectory(k, n) for k in range(B)}
+        for j in range(1, NOUT + 1):
+            h.flux_network_eval()
+            h.advance_to(0.0 + (T - 0.0) * j / NOUT)
+            for k in range(B):
+                assert np.array_equal(h.get_state(k).astype(np.float32), frames[k][j]), (j, k)
+        steps = a.times()[1]
+        assert len(set(steps.tolist())) > 1  # the instances took different numbers of steps
+        for k in range(B):
+            assert a.time(k) == h.time(k)
+            assert np.array_equal(a.get_state(k), h.get_state(k))
