@@ -308,8 +308,19 @@ def rollout(model, x0, steps):
 
 
 def rollout_metrics(pred, true, std):
-    """Per frame and variable RMSE over std, and density L1 (rollout.py:54-62).  pred, true physical
-    (T, 4, n, n); std (4,)."""
-    rmse = torch.sqrt(torch.mean((pred - true) ** 2, dim=(-2, -1))) / torch.as_tensor(std, device=pred.device)
+    """Per-frame rollout metrics of surrogate/rollout.py:54-62 plus density L1.  pred, true: physical
+    (T, 4, n, n) frames 1..T of a rollout and of its truth; std (4,).  Returns a dict of
+      rmse      (T, 4) RMSE per variable over the variable's std
+      l1_rho    (T,)   mean |rho_pred - rho_true|
+      drift     (T, 4) change of each total (mass, x/y momentum, energy) from the truth's first frame,
+                       over that frame's total absolute amount (net momentum can be ~0)
+      bad_cells (T,)   cells with density <= 0 or a pressure that is not > 0 (NaN counts as bad)."""
+    std = torch.as_tensor(std, device=pred.device, dtype=pred.dtype)
+    rmse = torch.sqrt(torch.mean((pred - true) ** 2, dim=(-2, -1))) / std
     l1_rho = torch.mean(torch.abs(pred[:, 0] - true[:, 0]), dim=(-2, -1))
-    return rmse, l1_rho
+    totals = pred.sum(dim=(-2, -1))
+    drift = (totals - true[0].sum(dim=(-2, -1))) / true[0].abs().sum(dim=(-2, -1))
+    rho, mx, my, E = pred[:, 0], pred[:, 1], pred[:, 2], pred[:, 3]
+    p = (GAMMA - 1.0) * (E - 0.5 * (mx * mx + my * my) / rho)
+    bad = ((rho <= 0) | ~(p > 0)).sum(dim=(-2, -1))
+    return {"rmse": rmse, "l1_rho": l1_rho, "drift": drift, "bad_cells": bad}

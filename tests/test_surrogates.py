@@ -159,7 +159,8 @@ def test_training_losses_and_rollout():
         assert len(hist) == 5 and all(np.isfinite(hist))
     out = M.rollout(m, norm.encode(frames[:, 0]), 4)
     assert out.shape == (3, 5, 4, 16, 16)
-    rmse, l1 = M.rollout_metrics(norm.decode(out[0]), frames[0, :5], [0.3] * 4)
+    m = M.rollout_metrics(norm.decode(out[0]), frames[0, :5], [0.3] * 4)
+    rmse, l1 = m["rmse"], m["l1_rho"]
     assert rmse.shape == (5, 4) and l1.shape == (5,)
 
 
@@ -184,3 +185,24 @@ def test_resnet_learns_solver_frames_on_gpu(npde_mod):
         err = float(torch.mean((model(x) - y) ** 2))
         persist = float(torch.mean((x - y) ** 2))
     assert err < 0.8 * persist, (err, persist)
+
+
+def test_rollout_drift_and_bad_cells_hand_computed():
+    """surrogate/rollout.py:54-62: drift of each total from the truth's first frame over its total absolute
+    amount, and the count of cells with rho <= 0 or p not > 0, on frames small enough to compute by hand."""
+    true = torch.ones(2, 4, 2, 2, dtype=torch.float64)
+    true[:, 1] = torch.tensor([[1.0, -1.0], [2.0, -2.0]])   # x momentum: total 0, absolute total 6
+    true[:, 3] = 3.0                                          # E = 3: p = 0.4 (3 - 0.5 (mx^2 + 1) / 1) > 0
+    pred = true.clone()
+    pred[1, 0, 0, 0] = 1.5                                    # mass 4 -> 4.5: drift +0.5 / 4
+    pred[1, 1, 1, 1] = -1.0                                   # x momentum 0 -> 1: drift 1 / 6
+    pred[1, 3, 0, 1] = 0.5                                    # p = 0.4 (0.5 - 1) < 0: one bad cell
+    pred[1, 0, 1, 0] = float("nan")                           # NaN density: bad (and mass drift NaN)
+    m = M.rollout_metrics(pred, true, [1.0] * 4)
+    d = m["drift"].numpy()
+    assert np.allclose(d[0], 0.0) and np.isnan(d[1, 0])
+    assert abs(d[1, 1] - 1.0 / 6.0) < 1e-15 and d[1, 2] == 0.0 and abs(d[1, 3] - (0.5 - 3.0) / 12.0) < 1e-15
+    assert m["bad_cells"].tolist() == [0, 2]
+    pred[1, 0, 1, 0] = 1.0
+    m = M.rollout_metrics(pred, true, [1.0] * 4)
+    assert abs(float(m["drift"][1, 0]) - 0.5 / 4.0) < 1e-15 and m["bad_cells"].tolist() == [0, 1]
