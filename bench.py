@@ -82,11 +82,147 @@ def cpu_baseline(U0, seconds, threads=None):
                       f"from t = 0 in {elapsed:.2f} s on {threads} OpenMP threads of {cpu_model()}"}
 
 
+LIB = ROOT / "neural-pde-surrogate_amd" / "lib" / "libnpde.so"
+DRIVER = ROOT / "neural-pde-surrogate_amd" / "lib" / "npde-euler2d"
+REF_CLI = ROOT / "oracle" / "_ref" / "euler2d"
+COUNTERS = ROOT / "profiles" / "stage_kernels_64x512_counters.md"
+
+
+def library_sha16():
+    import hashlib
+    return hashlib.sha256(LIB.read_bytes()).hexdigest()[:16] if LIB.exists() else None
+
+
+def counters_stale():
+    """True when the committed counter summary was profiled on another build of libnpde.so than the one
+    loaded here (its `library_sha16 = ...` line differs), so its per-launch figures may not describe the
+    kernels that ran."""
+    if not COUNTERS.exists():
+        return True
+    for line in COUNTERS.read_text().splitlines():
+        if line.startswith("library_sha16 = "):
+            return line.split("=", 1)[1].strip() != library_sha16()
+    return True
+
+
+def done_line(out):
+    """(steps, million cell-updates/s) from an euler2d-style "Done:" line (main2d.cpp:259-260)."""
+    import re
+    m = re.search(r"Done: (\d+) steps in [0-9.]+ s \(([0-9.]+) million cell-updates per second\)", out)
+    return (int(m.group(1)), float(m.group(2))) if m else (None, None)
+
+
+def cli_run(exe, args, threads=None):
+    import subprocess
+    import tempfile
+    env = dict(os.environ)
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    with tempfile.TemporaryDirectory() as d:
+        p = subprocess.run([str(exe), *args, "--out", d], capture_output=True, text=True, env=env, timeout=300)
+    steps, mcells = done_line(p.stdout)
+    return {"rc": p.returncode, "steps": steps, "value": None if mcells is None else mcells * 1e6,
+            "unit": "cell-updates/s"}
+
+
+def timed_trajectory(b, ics, t_end, nout, sizes, upload=False, download=False):
+    """Seconds of a recorded run of the batch from the states ics after a short untimed run with the same
+    frame count and sizes (buffers and captured graphs kept): the device-resident loop alone, or with the
+    state uploads before it and the trajectory downloads after it (end to end)."""
+    import torch
+    for k, U in enumerate(ics):
+        b.set_state(k, U)
+    b.run_trajectory(t_end / 50, nout, sizes=sizes)
+    if not upload:
+        for k, U in enumerate(ics):
+            b.set_state(k, U)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if upload:
+        for k, U in enumerate(ics):
+            b.set_state(k, U)
+    b.run_trajectory(t_end, nout, sizes=sizes)
+    if download:
+        for k in range(len(ics)):
+            for m in sizes:
+                b.trajectory(k, m)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def config_runs(cpu=True):
+    """The benchmark's other workloads, each at its stated size, as cell-update rates (one cell-update = one
+    cell advanced one SSP-RK2 step, main2d.cpp:258):
+      lax_liu3_256   the reference's own CPU case (Lax-Liu configuration 3, 256^2, t = 0.3, 10 snapshots;
+                     682 steps, experiments/results.md:92-94): the device loop in this process, the drop-in
+                     driver npde-euler2d end to end (snapshot files included, its "Done:" line), and the
+                     reference CLI built from its sources on all host threads and on one
+      riemann_64x512 the benchmark batch end to end: 64 random-Riemann instances of 512^2 to t = 0.5 with 50
+                     snapshots at 128^2 and 64^2 and each instance's own tail (uploads and downloads
+                     included), and its device loop alone
+      blast_96       the uneven batch: 96 blast instances of 128^2-1024^2 (4:3:2:1) to t = 0.2 with 50
+                     snapshots at 128^2 and 64^2
+      kh_1024        one held-out Kelvin-Helmholtz instance at 1024^2 (600 steps from t = 0)"""
+    import numpy as np
+    import npde
+    from npde.families import blast, blast_size, kh, riemann
+    out = {}
+    g, t_end, U0 = npde.builtin_problem("lax_liu3", 256)
+    with npde.Batch([g], "hllc", "vanleer", 0.5) as b:
+        secs = timed_trajectory(b, [U0], t_end, 10, ())
+        steps = int(b.time(0)[1])
+    ll = {"steps": steps, "device_loop": {"value": 256 * 256 * steps / secs, "seconds": secs}}
+    if DRIVER.exists():
+        ll["driver_end_to_end"] = cli_run(DRIVER, ["--test", "lax_liu3", "--n", "256"])
+    if cpu and REF_CLI.exists():
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        ll["reference_cli"] = dict(cli_run(REF_CLI, ["--test", "lax_liu3", "--n", "256"], threads=cores), cores=cores)
+        ll["reference_cli_1_thread"] = cli_run(REF_CLI, ["--test", "lax_liu3", "--n", "256"], threads=1)
+    out["lax_liu3_256"] = ll
+    log(f"lax_liu3 256^2: {steps} steps, device loop {ll['device_loop']['value'] / 1e9:.2f} G cell-updates/s")
+
+    n, B = 512, 64
+    ics = [riemann(k, n)[0] for k in range(B)]
+    with npde.Batch([npde.make_grid(n, n, bc_x=1, bc_y=1)] * B, "hllc", "vanleer", 0.5) as b:
+        e2e = timed_trajectory(b, ics, 0.5, 50, (128, 64), upload=True, download=True)
+        cu = float(np.sum(b.times()[1])) * n * n
+        dev = timed_trajectory(b, ics, 0.5, 50, (128, 64))
+    out["riemann_64x512"] = {"cell_updates": cu, "end_to_end": {"value": cu / e2e, "seconds": e2e},
+                             "device_loop": {"value": cu / dev, "seconds": dev}}
+    log(f"riemann 64 x 512^2 to t = 0.5: end to end {cu / e2e / 1e9:.2f} G/s, device loop {cu / dev / 1e9:.2f} G/s")
+
+    Bb = 96
+    sizes = [blast_size(k) for k in range(Bb)]
+    ics = [blast(k)[0] for k in range(Bb)]
+    with npde.Batch([npde.make_grid(m, m, bc_x=1, bc_y=1) for m in sizes], "hllc", "vanleer", 0.5) as b:
+        secs = timed_trajectory(b, ics, 0.2, 50, (128, 64))
+        st = b.times()[1]
+        cu = float(sum(m * m * int(k) for m, k in zip(sizes, st)))
+    out["blast_96"] = {"cell_updates": cu, "value": cu / secs, "seconds": secs,
+                       "steps_min": int(st.min()), "steps_max": int(st.max()),
+                       "sizes": {str(m): sizes.count(m) for m in (128, 256, 512, 1024)}}
+    log(f"blast 96 instances to t = 0.2: {cu / secs / 1e9:.2f} G/s in {secs:.2f} s")
+
+    import torch
+    U, _, _ = kh(0, 1024)
+    with npde.Batch([npde.make_grid(1024, 1024, bc_x=1, bc_y=1)], "hllc", "vanleer", 0.5) as b:
+        b.set_state(0, U)
+        b.advance_steps(32, 2.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.advance_steps(600, 2.0)
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+    out["kh_1024"] = {"value": 1024 * 1024 * 600 / secs, "seconds": secs, "steps": 600}
+    log(f"kh 1024^2 single instance: {out['kh_1024']['value'] / 1e9:.2f} G/s")
+    return out
+
+
 def counter(name, layout):
     """A per-launch stage-2 figure for this task layout from the committed rocprofv3 counter summary
     (lines `stage2_<name> (<layout>) = value`), or None without a measurement of that layout.  The
     HBM bytes are FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors a plain device copy of known size gives."""
-    f = ROOT / "profiles" / "stage_kernels_64x512_counters.md"
+    f = COUNTERS
     if not f.exists():
         return None
     key = f"stage2_{name} ({layout}) = "
@@ -106,7 +242,8 @@ def valu_roofline(layout, launch_ms):
     peak = 256 * 4 * 2.4e9 / 4
     rate = active / (launch_ms * 1e-3)
     out = {"bound": "fp64 valu issue", "achieved": rate, "peak": peak, "unit": "wave-instr/s", "frac": rate / peak,
-           "insts_per_launch": counter("valu_insts_per_launch", layout), "active_per_launch": active}
+           "insts_per_launch": counter("valu_insts_per_launch", layout), "active_per_launch": active,
+           "counters_stale": counters_stale()}
     flops = counter("fp64_flops_per_launch", layout)
     if flops is not None:
         out["fp64_tflops"] = flops / (launch_ms * 1e-3) / 1e12
@@ -201,6 +338,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-fast", action="store_true", help="skip the fast-arithmetic (not bit-exact) variant")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-solver time-to-matched-L1 evaluation")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other workloads (config_runs)")
     ap.add_argument("--rows", type=int, default=None, help="fix the band height (skips autotuning)")
     ap.add_argument("--wpb", type=int, default=None, help="fix the waves per block (skips autotuning)")
     ap.add_argument("--row-groups", type=int, default=None, choices=(0, 1, 2),
@@ -268,6 +406,7 @@ def main():
     fast = None if args.no_fast else fast_variant(b, ics, args.steps, args.warmup)
     b.close()
     learned = None if args.no_learned else learned_eval()
+    configs = None if args.no_configs else config_runs(cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
@@ -286,7 +425,8 @@ def main():
                    "instances": N_INST, "n": N, "tuning": tuning},
         "roofline": {"bound": "hbm", "kernel": f"npde::exact::k_stage<HLLC, vanLeer, 2, false, {layout}>",
                      "achieved": ach2, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach2 / HBM_PEAK_GBS,
-                     "traffic": counter("hbm_bytes_per_launch", layout), "bytes_per_cell": BYTES_STAGE2, "launch_ms": ms2,
+                     "traffic": counter("hbm_bytes_per_launch", layout), "library_sha16": library_sha16(),
+                     "traffic_stale": counters_stale(), "bytes_per_cell": BYTES_STAGE2, "launch_ms": ms2,
                      "stage1_launch_ms": ms1, "stage1_achieved": ach1,
                      # the whole step: every cell-update's algorithmic 160 B (stage 1 reads U and writes
                      # U1, stage 2 reads U and U1 and writes U) at the measured cell-update rate
@@ -297,6 +437,7 @@ def main():
         "cpu_baseline": cpu,
         "fast_arith_variant": fast,
         "learned_matched_l1": learned,
+        "configs": configs,
     }
     print(json.dumps(out), flush=True)
 

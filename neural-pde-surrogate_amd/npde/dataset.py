@@ -75,8 +75,13 @@ def pressure(U: np.ndarray) -> np.ndarray:
     return (GAMMA - 1) * (E - 0.5 * (mx ** 2 + my ** 2) / rho)
 
 
-def instance_size(family: str, index: int, riemann_n: int) -> int:
+REFERENCE_N = 512  # the reference dataset's solver size (data/generate.py; manifest solver_n)
+
+
+def instance_size(family: str, index: int, riemann_n: int, source: str = "benchmark") -> int:
     """Grid size of an instance, without building its initial condition."""
+    if source == "reference":
+        return REFERENCE_N
     if family == "riemann":
         return riemann_n
     if family == "blast":
@@ -84,8 +89,12 @@ def instance_size(family: str, index: int, riemann_n: int) -> int:
     return KH_N
 
 
-def instance(family: str, index: int, riemann_n: int):
-    """(U, n, t_start, t_end) of one instance."""
+def instance(family: str, index: int, riemann_n: int, source: str = "benchmark"):
+    """(U, n, t_start, t_end) of one instance: of the benchmark's families (npde.families), or with
+    source "reference" of the reference's dataset families at its solver size (REFERENCE_FAMILIES)."""
+    if source == "reference":
+        U, ts, te, _ = families.REFERENCE_FAMILIES[family](index, REFERENCE_N)
+        return U, REFERENCE_N, ts, te
     if family == "riemann":
         U, t_end, _ = families.riemann(index, riemann_n)
         return U, riemann_n, 0.0, t_end
@@ -98,17 +107,18 @@ def instance(family: str, index: int, riemann_n: int):
     raise ValueError(f"unknown family '{family}'")
 
 
-def generate(items, sizes, riemann_n=512, batch=256, log=print):
+def generate(items, sizes, riemann_n=512, batch=256, log=print, source="benchmark"):
     """items: [(family, index)].  Returns {(family, index): {"frames": {m: (51, 4, m, m) float32},
-    "t": (51,), "n": n, "steps": steps}} and the seconds spent."""
+    "t": (51,), "n": n, "steps": steps}} and the seconds spent.  source: "benchmark" or "reference"
+    (instance)."""
     out = {}
     t0 = time.perf_counter()
     # batches of like instances (same time window, same size) step for similar counts, so a batch does
     # not wait long for its slowest member; results are keyed by (family, index), so order is free
-    items = sorted(items, key=lambda it: (it[0] == "kh", instance_size(it[0], it[1], riemann_n)))
+    items = sorted(items, key=lambda it: (it[0] == "kh", instance_size(it[0], it[1], riemann_n, source)))
     for s0 in range(0, len(items), batch):
         chunk = items[s0:s0 + batch]
-        ics = [instance(f, i, riemann_n) for f, i in chunk]
+        ics = [instance(f, i, riemann_n, source) for f, i in chunk]
         # one batch per (t_start, t_end): instances of a batch share the frame schedule
         groups = {}
         for k, (U, n, ts, te) in enumerate(ics):
@@ -212,18 +222,35 @@ def main(argv=None):
     ap.add_argument("--blast", type=int, default=0, help="blast instances (indices 0..N-1)")
     ap.add_argument("--kh", type=int, default=0, help="kh instances (held out, 1024^2)")
     ap.add_argument("--sizes", default="128,64")
+    ap.add_argument("--families", choices=("benchmark", "reference"), default="benchmark",
+                    help="the benchmark's families, or the reference's dataset families at 512^2 (riemann and "
+                         "blast over t = 0 .. 0.25, kh over its own t_start + 0.25)")
+    ap.add_argument("--no-pack", action="store_true", help="generate (and check) only; write no files")
+    ap.add_argument("--check-steps", default=None, help="a step-count fixture (tests/golden/"
+                    "reference_manifest_steps.json): compare every generated trajectory's step count")
     # larger batches overlap more of their instances' tails: 160 instances took 10.8 s in batches of 64
     # and 10.1 s in one batch of 256
     ap.add_argument("--batch", type=int, default=256)
     args = ap.parse_args(argv)
     out = Path(args.out)
     out.mkdir(parents=True, exist_ok=True)
-    sizes = tuple(int(s) for s in args.sizes.split(","))
+    sizes = tuple(int(s) for s in args.sizes.split(",") if s)
     items = ([("riemann", i) for i in range(args.riemann)] + [("blast", i) for i in range(args.blast)]
              + [("kh", i) for i in range(args.kh)])
     print(f"generating {len(items)} trajectories", flush=True)
     trajs, secs = generate(items, sizes, riemann_n=args.riemann_n, batch=args.batch,
-                           log=lambda s: print(s, flush=True))
+                           log=lambda s: print(s, flush=True), source=args.families)
+    if args.check_steps:
+        rows = json.loads(Path(args.check_steps).read_text())["rows"]
+        want = {(r[0], r[1]): r[3] for r in rows}
+        got = {k: v["steps"] for k, v in trajs.items() if k in want}
+        same = sum(got[k] == want[k] for k in got)
+        print(f"step counts equal to the fixture's: {same} of {len(got)}", flush=True)
+        for k in sorted(got):
+            if got[k] != want[k]:
+                print(f"  {k[0]} {k[1]}: {got[k]} steps, fixture {want[k]}", flush=True)
+    if args.no_pack:
+        raise SystemExit(0)
     print(f"packing into {out}", flush=True)
     ok = pack(trajs, sizes, out, log=lambda s: print(s, flush=True))
     print("ALL CHECKS PASSED" if ok else "SOME CHECKS FAILED: see the lines above", flush=True)

@@ -123,3 +123,103 @@ def kh(index: int, n: int, family: str = "kh_b4"):
     p = np.full((n, n), 2.5)
     t_start = rng.uniform(0.5, 1.5)
     return to_conserved(rho, u, v, p), 2.0, {"ratio": ratio, "du": du, "amp": amp, "t_start": t_start}
+
+
+# ---------------------------------------------------------------------------------------------------
+# The reference's own dataset families (data/families.py), for datasets interchangeable with the
+# reference's packed files.  Same seeding (rng_for), same draws in the same order and the same per-cell
+# formulas as data/families.py:35-171 (densities, pressures and ratios log-uniform), with the cell centres
+# of data/ic.py:15-17 (lo + (i + 0.5) * dx).  Each returns (U, t_start, t_end, params) with U the conserved
+# planes (4, n, n) of the primitive fields the reference writes into its initial-condition file; its
+# driver converts them with to_conserved on reading (io_hdf5.hpp:175-206).  The recorded window is
+# t_start .. t_end with 50 frames (data/generate.py:60-75).
+
+def _ref_centres(n: int) -> np.ndarray:
+    return 0.0 + (np.arange(n) + 0.5) * ((1.0 - 0.0) / n)
+
+
+def _log_uniform(rng: np.random.Generator, lo: float, hi: float) -> float:
+    return float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+
+
+def _ref_disc(X, Y, cx, cy, radius, width=0.005):
+    """Smoothed indicator of the disc of the given radius around (cx, cy) on the periodic unit box."""
+    dx = (X - cx + 0.5) % 1.0 - 0.5
+    dy = (Y - cy + 0.5) % 1.0 - 0.5
+    r = np.sqrt(dx ** 2 + dy ** 2)
+    return 0.5 * (1.0 - np.tanh((r - radius) / width))
+
+
+def ref_riemann(index: int, n: int):
+    """data/families.py:35-51: corner (xs, ys) ~ U[0.3, 0.7]^2, per quadrant (rho, u, v, p) with rho, p
+    log-uniform in [0.1, 2] and u, v ~ U[-1, 1]; t_end 0.25."""
+    rng = rng_for("riemann", index)
+    xs, ys = rng.uniform(0.3, 0.7), rng.uniform(0.3, 0.7)
+    states = np.array([[_log_uniform(rng, 0.1, 2.0), rng.uniform(-1.0, 1.0), rng.uniform(-1.0, 1.0),
+                        _log_uniform(rng, 0.1, 2.0)] for _ in range(4)])
+    X, Y = np.meshgrid(_ref_centres(n), _ref_centres(n))
+    q = (X > xs).astype(int) + 2 * (Y > ys).astype(int)
+    U = to_conserved(*(states[q, k] for k in range(4)))
+    return U, 0.0, 0.25, {"split_x": xs, "split_y": ys, "states": states}
+
+
+def ref_blast(index: int, n: int):
+    """data/families.py:82-119: background rho0, p0 (log-uniform in [0.5, 2], [0.1, 0.5]); 1-3 blasts
+    (centre ~ U[0, 1]^2, radius ~ U[0.04, 0.12], pressure and density ratios log-uniform in [5, 40] and
+    [0.5, 2], added to the background); 0-2 density bubbles in pressure balance (radius ~ U[0.05, 0.15],
+    ratio log-uniform in [0.3, 3], multiplying the density); tanh edges of width 0.005; t_end 0.25."""
+    rng = rng_for("blast", index)
+    rho0, p0 = _log_uniform(rng, 0.5, 2.0), _log_uniform(rng, 0.1, 0.5)
+    X, Y = np.meshgrid(_ref_centres(n), _ref_centres(n))
+    rho, p = np.full((n, n), rho0), np.full((n, n), p0)
+    blasts, bubbles = [], []
+    for _ in range(rng.integers(1, 4)):
+        cx, cy = rng.uniform(0, 1, size=2)
+        radius = rng.uniform(0.04, 0.12)
+        pr, rr = _log_uniform(rng, 5.0, 40.0), _log_uniform(rng, 0.5, 2.0)
+        w = _ref_disc(X, Y, cx, cy, radius)
+        p += w * (pr - 1.0) * p0
+        rho += w * (rr - 1.0) * rho0
+        blasts.append([cx, cy, radius, pr, rr])
+    for _ in range(rng.integers(0, 3)):
+        cx, cy = rng.uniform(0, 1, size=2)
+        radius = rng.uniform(0.05, 0.15)
+        br = _log_uniform(rng, 0.3, 3.0)
+        rho *= 1.0 + _ref_disc(X, Y, cx, cy, radius) * (br - 1.0)
+        bubbles.append([cx, cy, radius, br])
+    z = np.zeros((n, n))
+    return to_conserved(rho, z, z, p), 0.0, 0.25, {"rho0": rho0, "p0": p0, "blasts": blasts, "bubbles": bubbles}
+
+
+def ref_kh(index: int, n: int):
+    """data/families.py:137-171: shear strip of a random phase y0, outside density log-uniform in [0.5, 2],
+    strip/outside ratio log-uniform in [0.33, 3], shear ~ U[0.5, 1.5], bulk drift ~ U[-0.3, 0.3], pressure
+    log-uniform in [1.5, 3.5], layer thickness ~ U[0.015, 0.04], 1-4 seed wavelengths of amplitude
+    ~ U[0.005, 0.02] and random phase; recorded from t_start ~ U[0.2, 1.5] for 0.25."""
+    rng = rng_for("kh", index)
+    rho1 = _log_uniform(rng, 0.5, 2.0)
+    rho2 = rho1 * _log_uniform(rng, 0.33, 3.0)
+    shear = rng.uniform(0.5, 1.5)
+    drift = rng.uniform(-0.3, 0.3)
+    p0 = _log_uniform(rng, 1.5, 3.5)
+    L = rng.uniform(0.015, 0.04)
+    waves = int(rng.integers(1, 5))
+    amp = rng.uniform(0.005, 0.02)
+    phase = rng.uniform(0, 2 * np.pi)
+    y0 = rng.uniform(0, 1)
+    t_start = rng.uniform(0.2, 1.5)
+    X, Y = np.meshgrid(_ref_centres(n), _ref_centres(n))
+    yy = (Y - y0) % 1.0
+    with np.errstate(over="ignore"):  # every branch is evaluated; the one kept is finite
+        s = np.where(yy < 0.25, 0.5 * np.exp((yy - 0.25) / L),
+                     np.where(yy < 0.5, 1.0 - 0.5 * np.exp((0.25 - yy) / L),
+                              np.where(yy < 0.75, 1.0 - 0.5 * np.exp((yy - 0.75) / L),
+                                       0.5 * np.exp((0.75 - yy) / L))))
+    rho = rho1 + (rho2 - rho1) * s
+    u = drift + 0.5 * shear * (1.0 - 2.0 * s)
+    v = amp * np.sin(2 * np.pi * waves * X + phase)
+    U = to_conserved(rho, u, v, np.full((n, n), p0))
+    return U, t_start, t_start + 0.25, {"waves": waves, "t_start": t_start, "y0": y0}
+
+
+REFERENCE_FAMILIES = {"riemann": ref_riemann, "blast": ref_blast, "kh": ref_kh}

@@ -49,7 +49,8 @@ def features(U: torch.Tensor) -> torch.Tensor:
 
 
 class FluxCorrector(torch.nn.Module):
-    def __init__(self, hidden: int = 32, hidden_layers: int = 4, out_scale: float = 0.02, seed: int = 0):
+    def __init__(self, hidden: int = 32, hidden_layers: int = 4, out_scale: float = 0.02, seed: int = 0,
+                 calls_per_frame: int = 1):
         super().__init__()
         chans = [IN_FEATURES] + [hidden] * hidden_layers + [8]
         g = torch.Generator().manual_seed(seed)
@@ -63,6 +64,9 @@ class FluxCorrector(torch.nn.Module):
             self.weights.append(torch.nn.Parameter(w))
             self.biases.append(torch.nn.Parameter(torch.zeros(cout)))
         self.out_scale = out_scale
+        # network evaluations per frame interval: the factors are recomputed at calls_per_frame equally
+        # spaced times inside each interval (the solver lands on them as on frame times)
+        self.calls_per_frame = int(calls_per_frame)
 
     def params(self):
         out = []
@@ -116,13 +120,14 @@ class FluxCorrector(torch.nn.Module):
     def state(self) -> dict:
         return {f"w{k}": w.detach().cpu() for k, w in enumerate(self.weights)} | \
                {f"b{k}": b.detach().cpu() for k, b in enumerate(self.biases)} | \
-               {"out_scale": torch.tensor(self.out_scale)}
+               {"out_scale": torch.tensor(self.out_scale), "calls_per_frame": torch.tensor(self.calls_per_frame)}
 
     def load(self, st: dict):
         for k in range(len(self.weights)):
             self.weights[k].data.copy_(torch.as_tensor(st[f"w{k}"]))
             self.biases[k].data.copy_(torch.as_tensor(st[f"b{k}"]))
         self.out_scale = float(st["out_scale"])
+        self.calls_per_frame = int(st["calls_per_frame"]) if "calls_per_frame" in st else 1
 
 
 TRAINED_WEIGHTS = __import__("pathlib").Path(__file__).resolve().parent / "weights" / "flux_corrector_riemann64.npz"

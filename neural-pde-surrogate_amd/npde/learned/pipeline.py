@@ -135,7 +135,10 @@ def train(data: torch.Tensor, iters=600, batch=32, unroll=8, lr=1e-3, warmup=50,
     N, F = data.shape[0], data.shape[1]
     n = data.shape[-1]
     dx = 1.0 / n
-    tf = torch.tensor(frame_times(F - 1), dtype=torch.float64, device=dev)
+    S = model.calls_per_frame
+    # the network's call times: S per frame interval, as the device loop lands on them (frame_times of
+    # S x the frame count: the frame times themselves are bitwise among them)
+    tfs = torch.tensor(frame_times((F - 1) * S), dtype=torch.float64, device=dev)
     scales = SCALES.to(dev).view(1, 4, 1, 1)
     hist, ratios, skipped = [], [], 0
     t0 = time.perf_counter()
@@ -145,18 +148,19 @@ def train(data: torch.Tensor, iters=600, batch=32, unroll=8, lr=1e-3, warmup=50,
         if sym is not None:
             frames = dihedral(frames, sym)
         U = frames[:, 0]
-        t = tf[k0.to(dev)]
+        kd = k0.to(dev)
+        t = tfs[kd * S]
         loss = 0.0
         for j in range(1, nu + 1):
-            tn = tf[(k0 + j).to(dev)]
-
-            def interval(U_, t_, tn_=tn):
-                if corrected:
-                    cx, cy = model(U_)
-                    U2, t2, _ = ts.advance(U_, t_, tn_, dx, dx, cx_corr=cx, cy_corr=cy)
-                else:
-                    U2, t2, _ = ts.advance(U_, t_, tn_, dx, dx)
-                return U2, t2
+            def interval(U_, t_, j_=j):
+                for s_ in range(1, S + 1):
+                    tn_ = tfs[(kd + j_ - 1) * S + s_]
+                    if corrected:
+                        cx, cy = model(U_)
+                        U_, t_, _ = ts.advance(U_, t_, tn_, dx, dx, cx_corr=cx, cy_corr=cy)
+                    else:
+                        U_, t_, _ = ts.advance(U_, t_, tn_, dx, dx)
+                return U_, t_
 
             if recompute and corrected:
                 U, t = torch.utils.checkpoint.checkpoint(interval, U, t, use_reentrant=False)
@@ -277,11 +281,12 @@ def learned_rollout_states(model: FluxCorrector, U0: np.ndarray, tf, force_fallb
     the fused kernel keeps on the device)."""
     B, n = U0.shape[0], U0.shape[-1]
     nout, t_end = len(tf) - 1, float(tf[-1])
+    calls = model.calls_per_frame  # the run lands calls x nout times; every calls-th landing is a frame
     rho_max = U0[:, 0].reshape(B, -1).max(axis=1)
     grids = [S.make_grid(n, n, bc_x=S.PERIODIC, bc_y=S.PERIODIC)] * B
     with S.Batch(grids, "hllc", "vanleer", 0.5) as b:
         b.set_flux_network(model)
-        secs = _timed_run(b, list(U0), t_end, nout, (n,), net=True)
+        secs = _timed_run(b, list(U0), t_end, nout * calls, (n,), net=True)
         t0 = time.perf_counter()
         t_now, _ = b.times()
         peak = b.flux_network_peak()
@@ -289,7 +294,7 @@ def learned_rollout_states(model: FluxCorrector, U0: np.ndarray, tf, force_fallb
         if force_fallback:
             failed[:] = True
         secs += time.perf_counter() - t0
-        frames = np.stack([b.trajectory(k, n) for k in range(B)])
+        frames = np.stack([b.trajectory(k, n)[::calls] for k in range(B)])
     return frames, secs, failed
 
 
@@ -449,6 +454,8 @@ def main(argv=None):
     ap.add_argument("--max-seconds", type=float, default=None, help="stop training after this many seconds")
     ap.add_argument("--out-scale", type=float, default=None, help="bound of the increments (fraction of the face "
                     "flux scale) for a new corrector")
+    ap.add_argument("--calls-per-frame", type=int, default=None, help="network evaluations per frame interval "
+                    "(a new corrector, or a change for the one given by --init)")
     ap.add_argument("--kh-eval", type=int, default=0, help="only evaluate the trained weights on this many held-out "
                     "Kelvin-Helmholtz instances at 128^2 (kh128_eval.json)")
     args = ap.parse_args(argv)
@@ -477,11 +484,17 @@ def main(argv=None):
     log(f"training: {args.iters} iterations, batch {args.batch}, unroll {args.unroll}")
     t0 = time.perf_counter()
     cur = lambda p: max(2, min(args.unroll, 2 ** (1 + int(3 * p))))  # 2, 4, 8 over thirds of the run
-    init = None if args.out_scale is None else FluxCorrector(out_scale=args.out_scale)
+    init = None
+    if args.out_scale is not None or args.calls_per_frame is not None:
+        init = FluxCorrector(out_scale=0.02 if args.out_scale is None else args.out_scale,
+                             calls_per_frame=args.calls_per_frame or 1)
     if args.init:  # continue a previous run: full unroll from the start
         from .model import load_trained
         init, cur = load_trained(args.init), None
+        if args.calls_per_frame:
+            init.calls_per_frame = args.calls_per_frame
         log(f"starting from {args.init}")
+    log(f"network calls per frame interval: {(init or FluxCorrector()).calls_per_frame}")
     model, hist = train(data64, iters=args.iters, batch=args.batch, unroll=args.unroll, lr=args.lr, log=log,
                         curriculum=cur, model=init, max_seconds=args.max_seconds)
     train_s = time.perf_counter() - t0

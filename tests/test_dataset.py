@@ -108,9 +108,10 @@ def test_packed_files_are_deterministic(tmp_path):
 
 
 @pytest.mark.gpu
-def test_pipeline_end_to_end(npde_mod, tmp_path):
+def test_pipeline_end_to_end(npde_mod, oracle, tmp_path):
     """Ten riemann instances at 64^2 packed at 32^2 and 16^2: splits 8/1/1, every check passes, and the
-    packed frames and frame times equal a direct device-loop run of the same instance bit for bit."""
+    packed frames and frame times of an instance equal the oracle's run of it bit for bit (and a direct
+    device-loop run)."""
     from npde import dataset as D
     from npde.families import riemann
     items = [("riemann", i) for i in range(10)]
@@ -125,6 +126,36 @@ def test_pipeline_end_to_end(npde_mod, tmp_path):
         b.run_trajectory(0.5, 50, sizes=(32,))
         assert np.array_equal(U[3], b.trajectory(0, 32))
         assert np.array_equal(t[3], b.frame_times(0))
+    import oracle2d
+    a = np.ascontiguousarray(riemann(3, 64)[0])
+    ro = oracle.run(oracle2d.Grid.make(64, 64, bc_x="periodic", bc_y="periodic"), "hllc", "vanleer", 0.5, a, 0.5,
+                    nout=50, sizes=(32,))
+    assert U[3].tobytes() == ro["frames"][32].tobytes()
+    assert t[3].tobytes() == ro["frame_t"].tobytes()
+    assert man[3]["steps"] == ro["steps"]
     st = json.loads((tmp_path / "stats_16.json").read_text())
     u16 = h5(tmp_path / "train_16.h5", "U", "<f4").reshape(8, 51, 4, 16, 16).astype(np.float64)
     assert np.allclose(st["mean"], u16.transpose(2, 0, 1, 3, 4).reshape(4, -1).mean(axis=1), rtol=1e-12)
+
+
+@pytest.mark.gpu
+def test_reference_families_reproduce_the_manifest_step_counts(npde_mod):
+    """The reference's dataset families restated (npde.families.REFERENCE_FAMILIES: its seeding, draws and
+    formulas) and run through the GPU solver at 512^2 over their recorded windows take the step counts the
+    reference's own euler2d recorded in data/packed/manifest.json (tests/golden/reference_manifest_steps.json):
+    a sampler or solver that differed would change the wave speeds, hence the dt sequence and the counts.
+    The held-out family's first frame lands on its drawn start time, as the reference solver built from its
+    sources does (checked for kh 0: 0.719249643107025); the manifest's start times (the frame-0 times of the
+    authors' own runs, data/splits.py:58) sit within one ulp of it."""
+    from npde import dataset as D
+    rows = json.loads((ROOT / "tests" / "golden" / "reference_manifest_steps.json").read_text())["rows"]
+    by = {(r[0], r[1]): r for r in rows}
+    items = [("riemann", i) for i in range(24)] + [("blast", i) for i in range(24)] + [("kh", i) for i in (0, 1)]
+    trajs, _ = D.generate(items, (), batch=256, log=lambda *a: None, source="reference")
+    wrong = {k: (v["steps"], by[k][3]) for k, v in trajs.items() if v["steps"] != by[k][3]}
+    assert not wrong, wrong
+    from npde.families import ref_kh
+    for i in (0, 1):
+        t0 = trajs[("kh", i)]["t"][0]
+        assert t0 == ref_kh(i, 8)[1]
+        assert abs(t0 - by[("kh", i)][2]) <= np.spacing(t0)

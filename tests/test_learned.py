@@ -99,6 +99,16 @@ def test_dihedral_maps_commute_with_the_step():
     assert torch.equal(dihedral(dihedral(U0, g), inv), U0)
 
 
+def test_network_call_times_contain_the_frame_times_bitwise():
+    """With S network calls per frame interval the solver lands on frame_times(50 S); every S-th of those is
+    the frame time itself, bit for bit (0.5 S k and 0.5 k are exact, and both quotients are the correctly
+    rounded value of the same real number), so the learned frames are taken at the truth's times."""
+    from npde.learned.pipeline import frame_times
+    base = frame_times(50)
+    for S in (1, 2, 3, 4):
+        assert frame_times(50 * S)[::S] == base
+
+
 def test_fused_packing_places_every_weight():
     """pack_fused: fragment (layer, k-block b, n-tile t), lane l = 16 g + m, value j holds W[16 t + m][ci][tap]
     with (tap, ci) = (b, 8 g + j) for the 32-channel layers and (4 b + g, j) for the first layer; zero
