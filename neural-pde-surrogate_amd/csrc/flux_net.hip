@@ -42,17 +42,21 @@ __device__ __forceinline__ unsigned short bf16_bits(float f) {
     return (unsigned short)(u >> 16);
 }
 
-// tanh(y) = 1 - 2 / (exp(2y) + 1) on the hardware's base-2 exponential and reciprocal: finite for every
-// finite y (exp overflows to +inf: 1; underflows to 0: -1), within ~1e-7 of tanhf, far below the
-// bfloat16 rounding of the activations it produces.
-__device__ __forceinline__ float tanh_fast(float y) {
-    const float e = __builtin_amdgcn_exp2f(y * 2.8853900817779268f);  // exp(2y) = 2^(2y / ln 2)
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+// tanh-approximate GELU, 0.5 x (1 + tanh(y)) with y = sqrt(2/pi) (x + 0.044715 x^3), evaluated as the
+// equal x * sigmoid(2y) = x / (1 + exp(-2y)) on the hardware's base-2 exponential and reciprocal: finite
+// for every finite x (exp overflows to +inf: 0; underflows to 0: x), within ~1e-7 relative of the tanh
+// form, far below the bfloat16 rounding of the activations it produces.
+__device__ __forceinline__ float gelu_tanh(float x) {
+    const float z = x * __builtin_fmaf(0.0713548162726009f, x * x, 1.5957691216057308f);  // 2y
+    const float e = __builtin_amdgcn_exp2f(z * -1.4426950408889634f);                   // exp(-2y)
+    return x * __builtin_amdgcn_rcpf(1.0f + e);
 }
 
-__device__ __forceinline__ float gelu_tanh(float x) {
-    const float t = tanh_fast(0.7978845608028654f * (x + 0.044715f * x * x * x));
-    return 0.5f * x * (1.0f + t);
+// two floats -> two bfloat16 (round to nearest even) in one dword, low half first (v_cvt_pk_bf16_f32)
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned pack_bf16(float lo, float hi) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(f32x2{lo, hi}, bf16x2));
 }
 
 __device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
@@ -76,6 +80,9 @@ __device__ __forceinline__ void cell_features(double d, double m, double n, doub
 
 // One convolution layer of the stack: input region Win x Win (cells of `in`), output region Wout x Wout
 // (Wout = Win - 2) into `out` (hidden) or the face factors (head).  KB k-blocks of 32, NT n-tiles of 16.
+// The weight fragment is the MFMA's A operand (16 output channels x 32 k) and the cells' activations its
+// B operand (32 k x 16 cells), so lane (m, g) accumulates output channels 16t + 4g .. 16t + 4g + 3 of cell
+// 16 blk + m and writes them as one 8-byte record piece.
 //   FIRST: the input is the 8-feature region (k-block b, lane group g: tap 4b + g, 8 features; taps
 //          beyond the ninth are zero);
 //   HEAD:  8 output channels, out_scale * tanh written as binary64 factors into the instance's
@@ -93,14 +100,16 @@ __device__ __forceinline__ void net_layer(const unsigned short* __restrict__ in,
     for (int b = 0; b < KB; ++b)
 #pragma unroll
         for (int t = 0; t < NT; ++t) w[b][t] = as_bf16x8(frags[(b * NT + t) * kWave + lane]);
-    float bb[NT];  // the head has 8 biases: its lanes of channels 8-15 (no output) read none
+    float bb[NT][4];  // the head has 8 biases: lane groups 2 and 3 (channels 8-15, no output) read none
 #pragma unroll
-    for (int t = 0; t < NT; ++t) bb[t] = (HEAD && t * 16 + m >= 8) ? 0.f : bias[t * 16 + m];
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bb[t][i] = (HEAD && g >= 2) ? 0.f : bias[t * 16 + 4 * g + i];
 
     for (int blk = wave; blk < nblk; blk += kNetWaves) {
-        // this lane's A row: cell o of the output region (the last block's spare rows read the last cell)
-        int o = (blk << 4) + m;
-        o = o < ncell ? o : ncell - 1;
+        // this lane's cell (the last block's spare columns read the last cell and store nothing)
+        const int c = (blk << 4) + m;
+        const int o = c < ncell ? c : ncell - 1;
         const int oy = o / Wout, ox = o - oy * Wout;
         f32x4 acc[NT];
 #pragma unroll
@@ -118,28 +127,25 @@ __device__ __forceinline__ void net_layer(const unsigned short* __restrict__ in,
             }
             const bf16x8 av = as_bf16x8(a);
 #pragma unroll
-            for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, w[b][t], acc[t], 0, 0, 0);
+            for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[b][t], av, acc[t], 0, 0, 0);
         }
-        // D: lane holds cells 16 blk + 4g + i (i = 0..3) of output channel 16t + m
+        if (c >= ncell) continue;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int c = (blk << 4) + 4 * g + i;
-            if (c >= ncell) continue;
+        for (int t = 0; t < NT; ++t) {
+            if constexpr (HEAD) {
+                if (g < 2) {  // channels 4g .. 4g+3: the x-face factors (g = 0) or the y-face factors (g = 1)
+                    const int cy = c / Wout, cx = c - cy * Wout;
+                    double* dst = const_cast<double*>(g == 0 ? D.corr_x : D.corr_y) +
+                                  (long long)(gy0 + cy) * D.nx + gx0 + cx;
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const int ch = t * 16 + m;
-                const float v = acc[t][i] + bb[t];
-                if constexpr (HEAD) {
-                    if (ch < 8) {
-                        const float r = tanhf(v) * out_scale;
-                        const int cy = c / Wout, cx = c - cy * Wout;
-                        const long long off = (long long)(gy0 + cy) * D.nx + gx0 + cx + (ch & 3) * D.plane;
-                        double* dst = const_cast<double*>(ch < 4 ? D.corr_x : D.corr_y);
-                        dst[off] = double(r);
-                    }
-                } else {
-                    out[chunk_at(c, ch >> 3) + (ch & 7)] = bf16_bits(gelu_tanh(v));
+                    for (int i = 0; i < 4; ++i) dst[i * D.plane] = double(tanhf(acc[t][i] + bb[t][i]) * out_scale);
                 }
+            } else {
+                const float v0 = gelu_tanh(acc[t][0] + bb[t][0]), v1 = gelu_tanh(acc[t][1] + bb[t][1]);
+                const float v2 = gelu_tanh(acc[t][2] + bb[t][2]), v3 = gelu_tanh(acc[t][3] + bb[t][3]);
+                typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+                *reinterpret_cast<u32x2*>(out + chunk_at(c, 2 * t + (g >> 1)) + 4 * (g & 1)) =
+                    u32x2{pack_bf16(v0, v1), pack_bf16(v2, v3)};
             }
         }
     }
@@ -173,11 +179,7 @@ __device__ __forceinline__ void net_tile(const InstDesc& D, const NetTask& tk, u
         cell_features(d, D.U[off + P], D.U[off + 2 * P], D.U[off + 3 * P], f);
         // NaN compares false and leaves dmax alone: a non-finite state fails the solver's own check
         if (ry >= 5 && ry < 5 + kNetTile && rx >= 5 && rx < 5 + kNetTile && float(d) > dmax) dmax = float(d);
-        u32x4 rec;
-        rec.x = unsigned(bf16_bits(f[0])) | (unsigned(bf16_bits(f[1])) << 16);
-        rec.y = unsigned(bf16_bits(f[2])) | (unsigned(bf16_bits(f[3])) << 16);
-        rec.z = unsigned(bf16_bits(f[4])) | (unsigned(bf16_bits(f[5])) << 16);
-        rec.w = unsigned(bf16_bits(f[6])) | (unsigned(bf16_bits(f[7])) << 16);
+        const u32x4 rec{pack_bf16(f[0], f[1]), pack_bf16(f[2], f[3]), pack_bf16(f[4], f[5]), pack_bf16(f[6], f[7])};
         *reinterpret_cast<u32x4*>(bufB + (c << 3)) = rec;
     }
     // non-negative floats order as their bit patterns: one integer maximum per wavefront
