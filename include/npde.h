@@ -74,6 +74,8 @@ typedef struct {
                              two strip boundaries of its band (from edge columns the previous stage kernel
                              wrote): 1 (default) for batches of at least 2^18 cells, 2 at any size; 0: never
                              (60-column strips with a 2-column halo) */
+    int waves_per_simd;   /* occupancy cap of the stage kernels: at most this many resident wavefronts per SIMD,
+                             enforced by reserving LDS per workgroup (0: as many as registers and LDS allow) */
 } npde_tuning;
 
 typedef struct npde_solver npde_solver;
@@ -216,6 +218,9 @@ npde_status npde_set_arithmetic(npde_solver* s, int arith);
 npde_status npde_get_arithmetic(const npde_solver* s, int* arith);
 /* The task layout (npde_task_layout) the current tuning gives this handle's instances. */
 npde_status npde_get_task_layout(const npde_solver* s, int* layout);
+/* Resident wavefronts per SIMD the stage-2 kernel reaches under the current tuning (HIP's occupancy
+ * calculator on its registers, its LDS including an occupancy cap's reservation, and wavefront slots). */
+npde_status npde_get_occupancy(const npde_solver* s, int* waves_per_simd);
 /* Average device milliseconds of the stage-1 and stage-2 kernel launches over the next nsteps steps of
  * every instance, each with the CFL dt of its state as in npde_advance_steps, launched one at a time with
  * HIP events on the handle's stream around each stage kernel (the state advances). */

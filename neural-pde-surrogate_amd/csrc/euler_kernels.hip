@@ -18,8 +18,11 @@
 namespace npde {
 
 hipError_t launch_stage_fast(int scheme, int lim, int stage, bool corr, int waves_per_block, int mode, int row_waves,
-                             int compact, int seam_rows, const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks,
-                             Partial* part, int* ndone, hipStream_t s);
+                             int compact, int seam_rows, unsigned seam_bytes, unsigned lds_reserve,
+                             const InstDesc* desc, Ctl* ctl, const Task* tasks, int ntasks, Partial* part, int* ndone,
+                             hipStream_t s);
+hipError_t stage_occupancy_fast(int scheme, int lim, bool corr, int waves_per_block, int mode, int row_waves,
+                                unsigned seam_bytes, unsigned lds_reserve, int* workgroups);
 
 
 // ---------------------------------------------------------------------------------------------
@@ -346,19 +349,42 @@ static inline bool valid_mode(int mode, int row_waves) {
 // seam mode: compact = read the stage input's edge columns from the edge arrays (the previous stage
 // kernel of this launch sequence wrote them); seam_rows = the band height (at most kMaxSeamRows), which
 // sizes each wavefront's LDS rows
+// lds_reserve: LDS bytes each workgroup is to hold in total (an occupancy cap: a CU then holds only as many
+// workgroups as fit), or 0
 hipError_t launch_stage(int scheme, int lim, int stage, bool corr, bool fast, int waves_per_block, int mode,
-                        int row_waves, int compact, int seam_rows, const InstDesc* desc, Ctl* ctl, const Task* tasks,
-                        int ntasks, Partial* part, int* ndone, hipStream_t s) {
+                        int row_waves, int compact, int seam_rows, unsigned lds_reserve, const InstDesc* desc,
+                        Ctl* ctl, const Task* tasks, int ntasks, Partial* part, int* ndone, hipStream_t s) {
     if (!valid_mode(mode, row_waves)) return hipErrorInvalidValue;
     if (mode == kSeams && (seam_rows < 1 || seam_rows > kMaxSeamRows)) return hipErrorInvalidValue;
+    const unsigned seam = seam_lds_bytes(mode, waves_per_block, seam_rows);
     if (fast)
-        return launch_stage_fast(scheme, lim, stage, corr, waves_per_block, mode, row_waves, compact, seam_rows, desc,
-                                 ctl, tasks, ntasks, part, ndone, s);
+        return launch_stage_fast(scheme, lim, stage, corr, waves_per_block, mode, row_waves, compact, seam_rows, seam,
+                                 lds_reserve, desc, ctl, tasks, ntasks, part, ndone, s);
     dim3 grid, block;
     launch_dims(ntasks, waves_per_block, row_waves, grid, block);
-    hipLaunchKernelGGL(pick_stage(scheme, lim, stage, corr, mode), grid, block, seam_lds_bytes(mode, waves_per_block, seam_rows),
-                       s, desc, ctl, tasks, ntasks, part, ndone, compact, seam_rows);
+    const StageFn fn = pick_stage(scheme, lim, stage, corr, mode);
+    unsigned lds = 0;
+    const hipError_t e = stage_dynamic_lds(fn, seam, lds_reserve, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fn, grid, block, lds, s, desc, ctl, tasks, ntasks, part, ndone, compact, seam_rows);
     return hipGetLastError();
+}
+
+// Workgroups of the stage-2 kernel one CU holds at once with the launch configuration launch_stage uses
+// (HIP's occupancy calculator: registers, LDS including the reservation, wavefront slots).
+hipError_t stage_occupancy(int scheme, int lim, bool corr, bool fast, int waves_per_block, int mode, int row_waves,
+                           int seam_rows, unsigned lds_reserve, int* workgroups) {
+    if (!valid_mode(mode, row_waves)) return hipErrorInvalidValue;
+    const unsigned seam = seam_lds_bytes(mode, waves_per_block, seam_rows);
+    if (fast)
+        return stage_occupancy_fast(scheme, lim, corr, waves_per_block, mode, row_waves, seam, lds_reserve,
+                                    workgroups);
+    const StageFn fn = pick_stage(scheme, lim, 2, corr, mode);
+    unsigned lds = 0;
+    const hipError_t e = stage_dynamic_lds(fn, seam, lds_reserve, lds);
+    if (e != hipSuccess) return e;
+    const int threads = (row_waves > 0 ? row_waves : waves_per_block) * kWave;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(workgroups, reinterpret_cast<const void*>(fn), threads, lds);
 }
 
 hipError_t launch_wavespeed(int waves_per_block, int mode, int row_waves, const InstDesc* desc, Ctl* ctl,

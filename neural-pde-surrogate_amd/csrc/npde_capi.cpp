@@ -24,8 +24,10 @@
 
 namespace npde {
 hipError_t launch_stage(int scheme, int lim, int stage, bool corr, bool fast, int waves_per_block, int mode,
-                        int row_waves, int compact, int seam_rows, const InstDesc* desc, Ctl* ctl, const Task* tasks,
-                        int ntasks, Partial* part, int* ndone, hipStream_t s);
+                        int row_waves, int compact, int seam_rows, unsigned lds_reserve, const InstDesc* desc,
+                        Ctl* ctl, const Task* tasks, int ntasks, Partial* part, int* ndone, hipStream_t s);
+hipError_t stage_occupancy(int scheme, int lim, bool corr, bool fast, int waves_per_block, int mode, int row_waves,
+                           int seam_rows, unsigned lds_reserve, int* workgroups);
 hipError_t launch_wavespeed(int waves_per_block, int mode, int row_waves, const InstDesc* desc, Ctl* ctl,
                             const Task* tasks, int ntasks, Partial* part, int* ndone, hipStream_t s);
 hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int nftasks, int* ndone,
@@ -81,7 +83,8 @@ struct npde_solver {
     bool fast = false;  // NPDE_ARITH_FAST: the fast-arithmetic stage kernels
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    npde_tuning tune{0, 4, 32, 1, 1, 1};
+    npde_tuning tune{0, 4, 32, 1, 1, 1, 0};
+    int lds_per_cu = 65536;    // LDS bytes per CU of the handle's device (queried at creation)
     int layout = kStrips;  // StageMode of the current task list
     int row_waves = 0;   // row groups: wavefronts per band
     int band = 0;        // band height of the current task list
@@ -427,14 +430,27 @@ npde_status fill_bad(npde_solver* s, npde_bad_cell* bad) {
     return NPDE_OK;
 }
 
+// The occupancy cap (tune.waves_per_simd > 0): the LDS bytes each stage workgroup holds so that a CU fits
+// at most 4 * cap / (wavefronts per workgroup) workgroups -- at most `cap` wavefronts on each of its 4
+// SIMDs -- or 0.  w workgroups fit and w + 1 do not when each holds floor(LDS / w) rounded down to 512
+// bytes (for w <= 17).  Reservations above kMaxLaunchLds are refused by npde_set_tuning.
+constexpr unsigned kMaxLaunchLds = 65536;
+unsigned lds_reserve_for(const npde_solver* s, const npde_tuning& t, int row_waves) {
+    if (t.waves_per_simd <= 0) return 0;
+    const int wg = row_waves > 0 ? row_waves : t.waves_per_block;
+    const int fit = std::max(1, 4 * t.waves_per_simd / wg);
+    return unsigned(s->lds_per_cu / fit) & ~511u;
+}
+unsigned lds_reserve(const npde_solver* s) { return lds_reserve_for(s, s->tune, s->row_waves); }
+
 // One stage launch.  In seam mode the wavefronts' seam solves read the stage input's edge columns from
 // the edge arrays, except in stage 1 of the first step of a launch sequence (first_step: a host call, a
 // graph, an iteration loop), whose input the caller may have changed since a stage 2 last wrote them.
 npde_status launch_stage_kernel(npde_solver* s, int stage, bool first_step) {
     const int compact = (stage == 2 || !first_step) ? 1 : 0;
     HIPCHK(s, launch_stage(s->flux, s->lim, stage, s->has_corr, s->fast, s->tune.waves_per_block, s->layout,
-                           s->row_waves, compact, s->band, s->ddesc, s->dctl, s->dtasks, int(s->htasks.size()),
-                           s->dpart, s->dndone, s->stream));
+                           s->row_waves, compact, s->band, lds_reserve(s), s->ddesc, s->dctl, s->dtasks,
+                           int(s->htasks.size()), s->dpart, s->dndone, s->stream));
     return NPDE_OK;
 }
 
@@ -668,6 +684,9 @@ npde_status npde_batch_create(int n, const npde_grid_desc* grids, int flux, int 
         return st;
     };
     if (hipGetDevice(&s->device) != hipSuccess) return cleanup(NPDE_ERR_DEVICE);
+    if (hipDeviceGetAttribute(&s->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, s->device) !=
+            hipSuccess || s->lds_per_cu <= 0)
+        return cleanup(NPDE_ERR_DEVICE);
     if (bind_scratch(s) != NPDE_OK) return cleanup(NPDE_ERR_DEVICE);
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(NPDE_ERR_DEVICE);
     if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
@@ -1179,10 +1198,15 @@ npde_status npde_get_tuning(const npde_solver* s, npde_tuning* t) {
 npde_status npde_set_tuning(npde_solver* s, const npde_tuning* t) {
     if (!s || !t) return NPDE_ERR_INVALID;
     if (t->rows_per_task < 0 || !(t->waves_per_block == 1 || t->waves_per_block == 2 || t->waves_per_block == 4) ||
-        t->steps_per_graph < 1)
+        t->steps_per_graph < 1 || t->waves_per_simd < 0 || t->waves_per_simd > 8)
         return fail(s, NPDE_ERR_INVALID, "bad tuning");
+    // an occupancy cap must be reachable with a launch's LDS (checked for the layout it gives; row groups take
+    // their workgroup size from the width)
+    if (t->waves_per_simd > 0 && lds_reserve_for(s, *t, row_group_waves_for(s, *t)) > kMaxLaunchLds)
+        return fail(s, NPDE_ERR_INVALID, "occupancy cap below what a workgroup's LDS can enforce");
     const bool layout_changed = t->rows_per_task != s->tune.rows_per_task || t->row_groups != s->tune.row_groups ||
                                 t->seams != s->tune.seams;
+    // (the occupancy cap only changes the launches' LDS reservation: the captured graphs are dropped below)
     s->tune = *t;
     drop_graph(s);
     if (s->subset) return list_all(s);
@@ -1210,6 +1234,17 @@ npde_status npde_get_task_layout(const npde_solver* s, int* layout) {
                       int(NPDE_TASKS_SEAMS) == kSeams,
                   "npde_task_layout mirrors StageMode");
     *layout = s->layout;
+    return NPDE_OK;
+}
+
+npde_status npde_get_occupancy(const npde_solver* s, int* waves_per_simd) {
+    if (!s || !waves_per_simd) return NPDE_ERR_INVALID;
+    int wg = 0;
+    const hipError_t e = stage_occupancy(s->flux, s->lim, s->has_corr, s->fast, s->tune.waves_per_block, s->layout,
+                                         s->row_waves, s->band, lds_reserve(s), &wg);
+    if (e != hipSuccess) return NPDE_ERR_DEVICE;
+    const int waves = s->row_waves > 0 ? s->row_waves : s->tune.waves_per_block;
+    *waves_per_simd = wg * waves / 4;
     return NPDE_OK;
 }
 
@@ -1314,9 +1349,10 @@ npde_status npde_builtin_problem(const char* test, int n, npde_grid_desc* grid, 
 }
 
 // Autotuner: times candidate (task layout, rows_per_task, waves_per_block) on the handle's own instances
-// with host-driven steps of the current CFL dt, restoring the state after every trial.  Every
-// candidate computes the same bits (the per-cell arithmetic does not depend on the launch shape);
-// only the time differs.
+// with host-driven steps of the current CFL dt, restoring the state after every trial, then occupancy
+// caps below the best candidate's natural occupancy (fewer resident wavefronts per SIMD: less cache
+// contention between them, less latency hiding).  Every candidate computes the same bits (the per-cell
+// arithmetic does not depend on the launch shape); only the time differs.
 npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* chosen) {
     if (!s) return NPDE_ERR_INVALID;
     std::ostringstream key;
@@ -1330,7 +1366,7 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
     if (cache_path) {
         std::ifstream in(cache_path);
         std::string line;
-        while (std::getline(in, line)) {  // "key rows waves_per_block [row_groups [seams]]"
+        while (std::getline(in, line)) {  // "key rows waves_per_block [row_groups [seams [waves_per_simd]]]"
             std::istringstream ls(line);
             std::string k;
             npde_tuning t = s->tune;
@@ -1339,7 +1375,10 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
             int v = 0;
             if (ls >> v) {
                 t.row_groups = v;
-                if (ls >> v) t.seams = v;
+                if (ls >> v) {
+                    t.seams = v;
+                    t.waves_per_simd = ls >> v ? v : 0;
+                }
             }
             npde_status st = npde_set_tuning(s, &t);
             if (st == NPDE_OK && chosen) *chosen = s->tune;
@@ -1382,33 +1421,49 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
     probe.seams = 2;
     if (seams_apply_for(s, probe)) layouts.push_back({0, 2});
     layouts.push_back({0, 0});
+    // the fastest of 5 timed steps (after one warm-up), each the second step of a sequence as in the
+    // device loop: robust to a stray slow step
+    auto trial = [&](const npde_tuning& t) {
+        if ((st = npde_set_tuning(s, &t)) != NPDE_OK) return;
+        double ms_min = 1e30;
+        for (int rep = 0; rep < 6 && st == NPDE_OK; ++rep) {
+            copy_state(false);
+            st = host_step(s, dts.data(), true);
+            if (st == NPDE_OK) st = host_step(s, dts.data(), false);
+            if (rep > 0) ms_min = std::min(ms_min, s->last_ms);
+        }
+        if (st == NPDE_OK && ms_min < best_ms) {
+            best_ms = ms_min;
+            best = t;
+        }
+    };
     for (const auto& lay : layouts)
     for (int r : rows_c) {
         if (st != NPDE_OK) break;
         if (r > max_ny && r != rows_c[0]) continue;
         if (lay.second && r > 64) continue;  // seam bands keep their seam solves in LDS: at most 64 rows
         for (int w : wpb_c) {
+            if (st != NPDE_OK) break;
             if (lay.first && w != wpb_c[0]) continue;  // a row group's workgroup size is fixed by the width
             npde_tuning t = orig;
             t.rows_per_task = r;
             t.waves_per_block = w;
             t.row_groups = lay.first;
             t.seams = lay.second;
-            if ((st = npde_set_tuning(s, &t)) != NPDE_OK) break;
-            // the fastest of 5 timed steps (after one warm-up), each the second step of a sequence as in
-            // the device loop: robust to a stray slow step
-            double ms_min = 1e30;
-            for (int rep = 0; rep < 6 && st == NPDE_OK; ++rep) {
-                copy_state(false);
-                st = host_step(s, dts.data(), true);
-                if (st == NPDE_OK) st = host_step(s, dts.data(), false);
-                if (rep > 0) ms_min = std::min(ms_min, s->last_ms);
-            }
-            if (st == NPDE_OK && ms_min < best_ms) {
-                best_ms = ms_min;
-                best = t;
-            }
+            t.waves_per_simd = 0;
+            trial(t);
         }
+    }
+    // occupancy caps below the best candidate's own occupancy (caps an LDS reservation cannot enforce at
+    // this workgroup size are refused by npde_set_tuning and skipped)
+    int natural = 0;
+    if (st == NPDE_OK && (st = npde_set_tuning(s, &best)) == NPDE_OK) st = npde_get_occupancy(s, &natural);
+    const npde_tuning uncapped = best;
+    for (int cap = natural - 1; cap >= 1 && st == NPDE_OK; --cap) {
+        npde_tuning t = uncapped;
+        t.waves_per_simd = cap;
+        if (npde_set_tuning(s, &t) != NPDE_OK) continue;
+        trial(t);
     }
     copy_state(false);
     (void)hipFree(save);
@@ -1424,7 +1479,7 @@ npde_status npde_autotune(npde_solver* s, const char* cache_path, npde_tuning* c
     if (st == NPDE_OK && cache_path) {
         std::ofstream out(cache_path, std::ios::app);
         out << key.str() << " " << best.rows_per_task << " " << best.waves_per_block << " " << best.row_groups
-            << " " << best.seams << "\n";
+            << " " << best.seams << " " << best.waves_per_simd << "\n";
     }
     if (chosen) *chosen = s->tune;
     return st;

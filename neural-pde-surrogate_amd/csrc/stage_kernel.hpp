@@ -532,6 +532,19 @@ StageFn pick_stage_t(int scheme, int lim, int stage, bool corr) {
 // layouts' (strips' stage 1 from 126 to 178 VGPRs, 4 to 2 wavefronts per SIMD).
 StageFn pick_stage_seams(int scheme, int lim, int stage, bool corr);
 
+// Dynamic LDS of a stage launch: the seam rows (seam bytes), or under an occupancy cap (reserve > 0: LDS
+// bytes a workgroup must hold in total) what tops the kernel's static LDS up to the reservation, if more.
+inline hipError_t stage_dynamic_lds(StageFn fn, unsigned seam, unsigned reserve, unsigned& bytes) {
+    bytes = seam;
+    if (reserve == 0) return hipSuccess;
+    hipFuncAttributes a{};
+    const hipError_t e = hipFuncGetAttributes(&a, reinterpret_cast<const void*>(fn));
+    if (e != hipSuccess) return e;
+    const unsigned fixed = unsigned(a.sharedSizeBytes);
+    if (reserve > fixed && reserve - fixed > bytes) bytes = reserve - fixed;
+    return hipSuccess;
+}
+
 #if defined(NPDE_SEAMS_UNIT)
 StageFn pick_stage_seams(int scheme, int lim, int stage, bool corr) {
     return pick_stage_t<kSeams>(scheme, lim, stage, corr);

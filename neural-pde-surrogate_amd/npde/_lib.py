@@ -27,7 +27,7 @@ class BadCell(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("rows_per_task", C.c_int), ("waves_per_block", C.c_int), ("steps_per_graph", C.c_int),
-                ("use_graph", C.c_int), ("row_groups", C.c_int), ("seams", C.c_int)]
+                ("use_graph", C.c_int), ("row_groups", C.c_int), ("seams", C.c_int), ("waves_per_simd", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -70,6 +70,7 @@ _SIGNATURES = [
     ("npde_set_arithmetic", C.c_int, [_H, C.c_int]),
     ("npde_get_arithmetic", C.c_int, [_H, C.POINTER(C.c_int)]),
     ("npde_get_task_layout", C.c_int, [_H, C.POINTER(C.c_int)]),
+    ("npde_get_occupancy", C.c_int, [_H, C.POINTER(C.c_int)]),
     ("npde_last_elapsed_ms", C.c_int, [_H, _D]),
     ("npde_stream", C.c_void_p, [_H]),
     ("npde_builtin_problem", C.c_int, [C.c_char_p, C.c_int, C.POINTER(GridDesc), _D, _D]),

@@ -368,6 +368,14 @@ class Batch:
         check(self._h, self._lib.npde_get_task_layout(self._h, C.byref(m)))
         return self.TASK_LAYOUTS[m.value]
 
+    @property
+    def occupancy(self) -> int:
+        """Resident wavefronts per SIMD of the stage-2 kernel under the current tuning (registers, LDS with an
+        occupancy cap's reservation, wavefront slots)."""
+        m = C.c_int()
+        check(self._h, self._lib.npde_get_occupancy(self._h, C.byref(m)))
+        return m.value
+
     def autotune(self, cache_path: str | None = None) -> dict:
         t = Tuning()
         check(self._h, self._lib.npde_autotune(self._h, cache_path.encode() if cache_path else None, C.byref(t)))

@@ -185,7 +185,8 @@ def test_autotune_mid_run_then_the_run_continues_exactly(npde_mod, oracle, tmp_p
     lines = cache.read_text().splitlines()
     assert len(lines) == 1
     key, fields = lines[0].split()[0], [int(v) for v in lines[0].split()[1:]]
-    assert fields == [chosen["rows_per_task"], chosen["waves_per_block"], chosen["row_groups"], chosen["seams"]]
+    assert fields == [chosen["rows_per_task"], chosen["waves_per_block"], chosen["row_groups"], chosen["seams"],
+                      chosen["waves_per_simd"]]
     with npde_mod.Batch([grid] * B, "hllc", "vanleer", 0.5) as b:
         for k, U in enumerate(ics):
             b.set_state(k, U)
@@ -201,6 +202,15 @@ def test_autotune_mid_run_then_the_run_continues_exactly(npde_mod, oracle, tmp_p
         b.advance_to(T1)
         a = ics[1].copy()
         ro = oracle.run(periodic(n), "hllc", "vanleer", 0.5, a, T1)
+        assert b.time(1) == (ro["t"], ro["steps"]) and same(b.get_state(1), a)
+        # a line with an occupancy cap: read back and enforced, same bits
+        cache.write_text(f"{key} 16 1 0 2 1\n")
+        t = b.autotune(str(cache))
+        assert (t["rows_per_task"], t["waves_per_block"], t["seams"], t["waves_per_simd"]) == (16, 1, 2, 1)
+        assert b.occupancy <= 1
+        for k, U in enumerate(ics):
+            b.set_state(k, U)
+        b.advance_to(T1)
         assert b.time(1) == (ro["t"], ro["steps"]) and same(b.get_state(1), a)
 
 

@@ -361,6 +361,40 @@ def test_launch_shape_does_not_change_bits(npde_mod, oracle, tuning):
         assert same(b.get_state(0), a)
 
 
+@pytest.mark.parametrize("tuning", [dict(waves_per_block=1, seams=0), dict(waves_per_block=2, seams=0),
+                                    dict(waves_per_block=1, seams=2, rows_per_task=16),
+                                    dict(waves_per_block=4, seams=2, rows_per_task=32)])
+def test_occupancy_cap_limits_resident_waves_and_keeps_bits(npde_mod, oracle, tuning):
+    """waves_per_simd caps the stage kernels' resident wavefronts by an LDS reservation: the occupancy calculator
+    sees at most the cap, caps an LDS reservation cannot enforce are refused, and every cap computes the same bits."""
+    g, t_end, U0 = npde_mod.builtin_problem("lax_liu3", 128)
+    go, _, _ = oracle.setup("lax_liu3", 128)
+    a = U0.copy()
+    ro = oracle.run(go, "hllc", "vanleer", 0.5, a, 0.02, nout=2)
+    with npde_mod.Batch([g], "hllc", "vanleer", 0.5) as b:
+        b.set_tuning(waves_per_simd=0, **tuning)
+        natural = b.occupancy
+        print(f"{tuning}: {natural} wavefronts per SIMD uncapped", flush=True)
+        assert 1 <= natural <= 8
+        accepted = []
+        for cap in range(1, natural):
+            try:
+                b.set_tuning(waves_per_simd=cap)
+            except npde_mod.NpdeError as e:
+                assert "occupancy cap" in str(e)
+                continue
+            assert b.occupancy <= cap
+            accepted.append(cap)
+            b.set_state(0, U0)
+            b.run_trajectory(0.02, 2, sizes=(64,))
+            assert b.time(0)[1] == ro["steps"]
+            assert same(b.get_state(0), a)
+        print(f"  caps enforced: {accepted}", flush=True)
+        assert natural == 1 or accepted or tuning["waves_per_block"] == 4  # (4-wave groups: caps 1, 2 need > 64 KiB)
+        b.set_tuning(waves_per_simd=0)
+        assert b.occupancy == natural
+
+
 def test_nonphysical_state_raises_with_first_bad_cell(npde_mod, golden):
     r = golden("runs.npz")
     g, _, U0 = npde_mod.builtin_problem("sod_x", 100)
