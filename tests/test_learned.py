@@ -424,3 +424,25 @@ def test_network_in_the_device_loop_equals_a_host_loop(npde_mod, cuda):
         for k in range(B):
             assert a.time(k) == h.time(k)
             assert np.array_equal(a.get_state(k), h.get_state(k))
+
+
+@pytest.mark.gpu
+def test_accuracy_cost_table(npde_mod, cuda):
+    """The accuracy-versus-cost table on a small setting (8 held-out instances, plain 32^2 and 64^2 and the
+    learned 64^2): every method has a positive cost and finite accuracy columns, conserves every total and has
+    no bad cell, the finer plain solve is the more accurate, and the Pareto flags mark exactly the methods
+    that no other method beats in both cost and last-frame error."""
+    from npde.learned import pareto
+    res = pareto.run(instances=8, ladder=(32, 64), log=lambda *a: None)
+    rows = {r["method"]: r for r in res["rows"]}
+    assert set(rows) == {"plain 32^2", "plain 64^2", "learned 64^2"}
+    for r in rows.values():
+        assert r["cost"] > 0 and np.isfinite(r["rmse_last"]) and np.isfinite(r["l1_rho"])
+        assert r["bad_cells"] == 0 and abs(r["drift_max"]) < 1e-4  # periodic: every total is conserved
+    assert rows["plain 64^2"]["rmse_last"] < rows["plain 32^2"]["rmse_last"]
+    assert rows["plain 64^2"]["l1_rho"] < rows["plain 32^2"]["l1_rho"]
+    assert any(r["pareto"] for r in rows.values())
+    for r in rows.values():
+        beaten = any(o is not r and o["cost"] <= r["cost"] and o["rmse_last"] <= r["rmse_last"]
+                     and (o["cost"] < r["cost"] or o["rmse_last"] < r["rmse_last"]) for o in rows.values())
+        assert r["pareto"] == (not beaten)
