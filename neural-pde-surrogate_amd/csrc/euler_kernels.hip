@@ -13,6 +13,7 @@
 // compiled with -ffp-contract=off.
 #include <hip/hip_runtime.h>
 
+#include "run_state.hpp"
 #include "stage_kernel.hpp"
 
 namespace npde {
@@ -24,51 +25,6 @@ hipError_t launch_stage_fast(int scheme, int lim, int stage, bool corr, int wave
 hipError_t stage_occupancy_fast(int scheme, int lim, bool corr, int waves_per_block, int mode, int row_waves,
                                 unsigned seam_bytes, unsigned lds_reserve, int* workgroups);
 
-
-// ---------------------------------------------------------------------------------------------
-// Run-state machine of one instance (executed by one lane).  Mirrors the driver loop
-// (main2d.cpp:224-246): targets T(0) = t_start, T(k) = t_start + (t_end - t_start) * k / nout; while
-// t < T(k) take a step with dt = min(cfl / (sx/dx + sy/dy), T(k) - t); on reaching T(k) write frame k
-// (if recording) and move to k + 1.  A non-physical state stops the instance when a step is needed
-// (the reference checks in compute_dt, solver2d.hpp:75-80).
-__device__ __forceinline__ double target_time(const Ctl* c, int k) {
-    if (k == 0) return c->t_start;
-    return c->t_start + (c->t_end - c->t_start) * double(k) / double(c->nout);
-}
-
-__device__ void decide_next(const InstDesc& D, Ctl* c, int* ndone) {
-    if (c->done) return;
-    for (;;) {
-        if (c->k > c->nout) {
-            c->done = 1;
-            atomicAdd(ndone, 1);
-            return;
-        }
-        const double T = target_time(c, c->k);
-        if (c->t < T) {
-            if (c->steps >= c->step_limit) {  // step budget of npde_advance_steps spent
-                c->done = 1;
-                atomicAdd(ndone, 1);
-                return;
-            }
-            if (c->bad != kNoBad) {
-                c->error = 1;
-                c->done = 1;
-                atomicAdd(ndone, 1);
-                return;
-            }
-            const double dt = D.cfl / (c->sx / D.dx + c->sy / D.dy);
-            c->dt = tmin(dt, T - c->t);
-            c->active = 1;
-            return;
-        }
-        if (c->record) {
-            c->pending = c->k;
-            return;
-        }
-        c->k += 1;
-    }
-}
 
 // Reduce an instance's per-task wave-speed partials (one wavefront), complete the step in flight and
 // decide the next one.  Runs in its own launch after the kernel that wrote the partials, so the
@@ -100,11 +56,10 @@ __device__ void finish_instance(const InstDesc& D, Ctl* c, const Partial* __rest
     }
 }
 
-// kAfterFrameHold: as kAfterFrame, but the instance then holds (flux-network runs) until k_release
-enum FinishMode : int { kAfterStep = 0, kAfterWavespeed = 1, kAfterFrame = 2, kAfterFrameHold = 3 };
+enum FinishMode : int { kAfterStep = 0, kAfterWavespeed = 1 };
 
-// One wavefront per instance: after a stage-2 launch (instances that stepped), after a wave-speed
-// launch (instances not done), or after a frame launch (instances whose frame was just written).
+// One wavefront per instance: after a stage-2 launch (instances that stepped) or after a wave-speed
+// launch (instances not done).
 __global__ __launch_bounds__(64) void k_finish(const InstDesc* __restrict__ desc, Ctl* __restrict__ ctl,
                                                const Partial* __restrict__ part, int n, int mode,
                                                int* __restrict__ ndone) {
@@ -113,47 +68,9 @@ __global__ __launch_bounds__(64) void k_finish(const InstDesc* __restrict__ desc
     Ctl* c = ctl + inst;
     const InstDesc& D = desc[inst];
     const int lane = threadIdx.x;
-    if (mode == kAfterFrame || mode == kAfterFrameHold) {
-        if (c->pending < 0 || lane != 0) return;
-        const int k = c->pending;
-        if (D.ftimes) D.ftimes[k] = c->t;
-        c->pending = -1;
-        c->k = k + 1;
-        // a flux-network run: the instance waits here for the batch's network call at this frame, unless
-        // that was its last frame
-        if (mode == kAfterFrameHold && c->k <= c->nout) {
-            c->hold = 1;
-            atomicAdd(ndone + kCountHeld, 1);
-            return;
-        }
-        decide_next(D, c, ndone);
-        return;
-    }
     const bool fresh = mode == kAfterStep ? c->active != 0 : c->done == 0;
     if (!fresh) return;
     finish_instance(D, c, part, lane, ndone);
-}
-
-// Flux-network runs: once every instance that is not done holds at a frame landing (and the network has
-// run for them), release them all -- each decides its next step -- and clear the count.  One workgroup,
-// so the count is read before any instance moves and cleared after all have.
-__global__ __launch_bounds__(256) void k_release(const InstDesc* __restrict__ desc, Ctl* __restrict__ ctl, int n,
-                                                 int* __restrict__ counters) {
-    __shared__ int go;
-    if (threadIdx.x == 0) {
-        const int held = counters[kCountHeld];
-        go = held > 0 && held == n - counters[kCountDone];
-    }
-    __syncthreads();
-    if (!go) return;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        Ctl* c = ctl + k;
-        if (!c->hold) continue;
-        c->hold = 0;
-        decide_next(desc[k], c, counters);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) counters[kCountHeld] = 0;
 }
 
 // Max wave speeds and first non-physical cell of U for every instance that is not done.  Same task
@@ -208,8 +125,11 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_wavespeed(const InstDe
 
 // Block averages of the conserved variables for instances with a pending frame
 // (trajectory.hpp:39-48: sum over the f x f block, rows then columns, from 0.0; times 1/(f*f); to float).
+// The last of an instance's frame workgroups to finish completes the landing (complete_landing; hold:
+// a flux-network run).  Every workgroup of the instance read `pending` before it counted itself in
+// Ctl::fdone, so the landing's update of the run state cannot overtake a reader.
 __global__ __launch_bounds__(256) void k_frame(const InstDesc* __restrict__ desc, Ctl* __restrict__ ctl,
-                                               const FTask* __restrict__ ftasks, int nftasks,
+                                               const FTask* __restrict__ ftasks, int nftasks, int hold,
                                                int* __restrict__ ndone) {
     const int bid = blockIdx.x;
     if (bid >= nftasks) return;
@@ -242,6 +162,11 @@ __global__ __launch_bounds__(256) void k_frame(const InstDesc* __restrict__ desc
         out[pl + cell] = float(s1 * inv);
         out[2 * pl + cell] = float(s2 * inv);
         out[3 * pl + cell] = float(s3 * inv);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && atomicAdd(&c->fdone, 1) == D.nftasks - 1) {
+        c->fdone = 0;
+        complete_landing(D, c, hold, ndone);
     }
 }
 
@@ -292,6 +217,7 @@ __global__ void k_begin_run(Ctl* __restrict__ ctl, int n, double t_start, double
     if (k == 0) {
         ndone[kCountDone] = 0;
         ndone[kCountHeld] = 0;
+        ndone[kCountNetGroups] = 0;
     }
     if (k >= n) return;
     Ctl& c = ctl[k];
@@ -401,10 +327,10 @@ hipError_t launch_wavespeed(int waves_per_block, int mode, int row_waves, const 
     return hipGetLastError();
 }
 
-hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int nftasks, int* ndone,
+hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int nftasks, int hold, int* ndone,
                         hipStream_t s) {
     if (nftasks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_frame, dim3(nftasks), dim3(256), 0, s, desc, ctl, ftasks, nftasks, ndone);
+    hipLaunchKernelGGL(k_frame, dim3(nftasks), dim3(256), 0, s, desc, ctl, ftasks, nftasks, hold, ndone);
     return hipGetLastError();
 }
 
@@ -420,10 +346,6 @@ hipError_t launch_block_average(const InstDesc* desc, int inst, int m, int nyc, 
     return hipGetLastError();
 }
 
-hipError_t launch_release(const InstDesc* desc, Ctl* ctl, int n, int* counters, hipStream_t s) {
-    hipLaunchKernelGGL(k_release, dim3(1), dim3(256), 0, s, desc, ctl, n, counters);
-    return hipGetLastError();
-}
 
 hipError_t launch_begin_run(Ctl* ctl, int n, double t_start, double t_end, int nout, int record, long long max_new_steps,
                             int* ndone, hipStream_t s) {

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include "npde_device.hpp"
+#include "run_state.hpp"
 
 namespace npde {
 
@@ -202,11 +203,13 @@ __device__ __forceinline__ void net_tile(const InstDesc& D, const NetTask& tk, u
 
 // Workgroups stride over the tile list.  counters == null: every instance's tiles.  Otherwise (the device
 // loop of a flux-network run) nothing happens until every instance that is not done holds at a frame
-// landing; then the tiles of the holding instances are evaluated (k_release lets them move on after).
-__global__ __launch_bounds__(kNetThreads) void k_flux_net(const InstDesc* __restrict__ desc,
-                                                          const Ctl* __restrict__ ctl,
+// landing; then the tiles of the holding instances are evaluated, and the last workgroup to finish
+// releases the holding instances (each decides its next step) and clears the counts.  Every workgroup
+// read the counts and the hold flags before it counted itself in counters[kCountNetGroups], so the
+// release cannot overtake a reader.
+__global__ __launch_bounds__(kNetThreads) void k_flux_net(const InstDesc* __restrict__ desc, Ctl* __restrict__ ctl,
                                                           const NetTask* __restrict__ tasks, int ntasks, int n,
-                                                          const int* __restrict__ counters,
+                                                          int* __restrict__ counters,
                                                           const u32x4* __restrict__ frags,
                                                           const float* __restrict__ bias, float out_scale,
                                                           unsigned* __restrict__ peak) {
@@ -216,6 +219,7 @@ __global__ __launch_bounds__(kNetThreads) void k_flux_net(const InstDesc* __rest
     }
     __shared__ __align__(16) unsigned short bufA[24 * 24 * 32];  // first and third layers' outputs
     __shared__ __align__(16) unsigned short bufB[22 * 22 * 32];  // features, second and fourth layers' outputs
+    __shared__ int last;
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const NetTask tk = tasks[t];
         if (counters && !ctl[tk.inst].hold) continue;
@@ -224,15 +228,29 @@ __global__ __launch_bounds__(kNetThreads) void k_flux_net(const InstDesc* __rest
         net_tile(D, tk, bufA, bufB, frags, bias, out_scale, peak);
         __syncthreads();  // the next tile's features overwrite the buffer the head read
     }
+    if (!counters) return;
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(counters + kCountNetGroups, 1) == int(gridDim.x) - 1;
+    __syncthreads();
+    if (!last) return;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        Ctl* c = ctl + k;
+        if (!c->hold) continue;
+        c->hold = 0;
+        decide_next(desc[k], c, counters);
+    }
+    if (threadIdx.x == 0) {
+        counters[kCountHeld] = 0;
+        counters[kCountNetGroups] = 0;
+    }
 }
 
 // counters: the run's counters in the device loop (see k_flux_net), or null for every instance.  At most
 // kNetBlocks workgroups (two per CU at this kernel's LDS footprint), so a launch that finds nothing to do
 // costs little.
 constexpr int kNetBlocks = 512;
-hipError_t launch_flux_net(const InstDesc* desc, const Ctl* ctl, const NetTask* tasks, int ntasks, int n,
-                           const int* counters, const void* frags, const float* bias, float out_scale, unsigned* peak,
-                           hipStream_t s) {
+hipError_t launch_flux_net(const InstDesc* desc, Ctl* ctl, const NetTask* tasks, int ntasks, int n, int* counters,
+                           const void* frags, const float* bias, float out_scale, unsigned* peak, hipStream_t s) {
     if (ntasks <= 0) return hipSuccess;
     const int blocks = ntasks < kNetBlocks ? ntasks : kNetBlocks;
     hipLaunchKernelGGL(k_flux_net, dim3(unsigned(blocks)), dim3(kNetThreads), 0, s, desc, ctl, tasks, ntasks, n,

@@ -30,17 +30,15 @@ hipError_t stage_occupancy(int scheme, int lim, bool corr, bool fast, int waves_
                            int seam_rows, unsigned lds_reserve, int* workgroups);
 hipError_t launch_wavespeed(int waves_per_block, int mode, int row_waves, const InstDesc* desc, Ctl* ctl,
                             const Task* tasks, int ntasks, Partial* part, int* ndone, hipStream_t s);
-hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int nftasks, int* ndone,
+hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int nftasks, int hold, int* ndone,
                         hipStream_t s);
 hipError_t launch_block_average(const InstDesc* desc, int inst, int m, int nyc, float* out, hipStream_t s);
 hipError_t launch_finish(const InstDesc* desc, Ctl* ctl, const Partial* part, int n, int mode, int* ndone,
                          hipStream_t s);
-enum : int { kAfterStep = 0, kAfterWavespeed = 1, kAfterFrame = 2, kAfterFrameHold = 3 };
+enum : int { kAfterStep = 0, kAfterWavespeed = 1 };
 hipError_t launch_prep_host_step(Ctl* ctl, int n, const double* dts, hipStream_t s);
-hipError_t launch_flux_net(const InstDesc* desc, const Ctl* ctl, const NetTask* tasks, int ntasks, int n,
-                           const int* counters, const void* frags, const float* bias, float out_scale, unsigned* peak,
-                           hipStream_t s);
-hipError_t launch_release(const InstDesc* desc, Ctl* ctl, int n, int* counters, hipStream_t s);
+hipError_t launch_flux_net(const InstDesc* desc, Ctl* ctl, const NetTask* tasks, int ntasks, int n, int* counters,
+                           const void* frags, const float* bias, float out_scale, unsigned* peak, hipStream_t s);
 hipError_t launch_mfma32_probe(const float* a, const float* b, float* d, hipStream_t s);
 hipError_t launch_begin_run(Ctl* ctl, int n, double t_start, double t_end, int nout, int record, long long max_new_steps,
                             int* ndone, hipStream_t s);
@@ -475,18 +473,15 @@ npde_status launch_wavespeed_and_finish(npde_solver* s) {
 npde_status launch_iterations(npde_solver* s, int iters, int nft) {
     for (int it = 0; it < iters; ++it) {
         if (nft > 0) {
-            HIPCHK(s, launch_frame(s->ddesc, s->dctl, s->dftasks, nft, s->dndone, s->stream));
-            if (s->net_frags) {
-                // flux-network run: an instance that reached its frame holds there; once every running
-                // instance holds, the network turns their states into the next interval's factors (one
-                // launch for the batch) and they all move on
-                HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterFrameHold, s->dndone, s->stream));
+            // the frame kernel writes the frames of the instances that landed and completes their landings
+            const int hold = s->net_frags ? 1 : 0;
+            HIPCHK(s, launch_frame(s->ddesc, s->dctl, s->dftasks, nft, hold, s->dndone, s->stream));
+            // flux-network run: an instance that reached its frame holds there; once every running instance
+            // holds, the network turns their states into the next interval's factors (one launch for the
+            // batch), and its last workgroup lets them all move on
+            if (hold)
                 HIPCHK(s, launch_flux_net(s->ddesc, s->dctl, s->dnettasks, s->nnettasks, s->n, s->dndone, s->net_frags,
                                           s->net_bias, s->net_scale, s->dnetpeak, s->stream));
-                HIPCHK(s, launch_release(s->ddesc, s->dctl, s->n, s->dndone, s->stream));
-            } else {
-                HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterFrame, s->dndone, s->stream));
-            }
         }
         npde_status st = launch_step_pair(s, it == 0);
         if (st != NPDE_OK) return st;

@@ -55,8 +55,10 @@ struct InstDesc {
     double* ftimes;              // time at which each frame was written (nout + 1), or null
 };
 
-// Mutable per-instance run state.  Written only by k_finish (one wavefront per instance) or by host
-// set-up between launches; read by every wavefront of the following launches.
+// Mutable per-instance run state.  Written by k_finish (one wavefront per instance), by the last frame
+// workgroup of an instance's landing (k_frame), by the last workgroup of a flux-network launch (releasing
+// the instances that held), or by host set-up between launches; read by every wavefront of the
+// following launches.
 struct Ctl {
     double t, dt, t_start, t_end;
     double sx, sy;        // max |u|+a and max |v|+a of the current state
@@ -72,12 +74,13 @@ struct Ctl {
     int error;            // 1: non-physical state met when a step was needed
     int bad;              // row-major index of the first non-physical cell, or kNoBad
     int hold;             // flux-network runs: landed on a frame, waiting for the batch's network call
+    int fdone;            // frame workgroups of the pending frame that have written their part (0 between launches)
 };
 
-// Device counters of a run, one int each: instances done, and (flux-network runs) instances holding at
-// a frame landing.  The network runs, and the holding instances move on, once every instance that is not
-// done holds.
-enum RunCounter : int { kCountDone = 0, kCountHeld = 1, kRunCounters = 2 };
+// Device counters of a run, one int each: instances done; (flux-network runs) instances holding at a
+// frame landing -- the network runs, and the holding instances move on, once every instance that is not
+// done holds -- and workgroups of that network launch that have finished (its last one releases them).
+enum RunCounter : int { kCountDone = 0, kCountHeld = 1, kCountNetGroups = 2, kRunCounters = 3 };
 
 struct Partial {
     double sx, sy;

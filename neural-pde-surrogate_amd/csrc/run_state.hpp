@@ -1,0 +1,71 @@
+// Run-state machine of a device-resident run, shared by the kernels that move an instance on: k_finish
+// (euler_kernels.hip), k_frame (euler_kernels.hip) and, in flux-network runs, k_flux_net (flux_net.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "euler_numerics.hpp"
+#include "npde_device.hpp"
+
+namespace npde {
+
+// Run-state machine of one instance (executed by one lane).  Mirrors the driver loop
+// (main2d.cpp:224-246): targets T(0) = t_start, T(k) = t_start + (t_end - t_start) * k / nout; while
+// t < T(k) take a step with dt = min(cfl / (sx/dx + sy/dy), T(k) - t); on reaching T(k) write frame k
+// (if recording) and move to k + 1.  A non-physical state stops the instance when a step is needed
+// (the reference checks in compute_dt, solver2d.hpp:75-80).
+__device__ __forceinline__ double target_time(const Ctl* c, int k) {
+    if (k == 0) return c->t_start;
+    return c->t_start + (c->t_end - c->t_start) * double(k) / double(c->nout);
+}
+
+__device__ inline void decide_next(const InstDesc& D, Ctl* c, int* ndone) {
+    if (c->done) return;
+    for (;;) {
+        if (c->k > c->nout) {
+            c->done = 1;
+            atomicAdd(ndone, 1);
+            return;
+        }
+        const double T = target_time(c, c->k);
+        if (c->t < T) {
+            if (c->steps >= c->step_limit) {  // step budget of npde_advance_steps spent
+                c->done = 1;
+                atomicAdd(ndone, 1);
+                return;
+            }
+            if (c->bad != kNoBad) {
+                c->error = 1;
+                c->done = 1;
+                atomicAdd(ndone, 1);
+                return;
+            }
+            const double dt = D.cfl / (c->sx / D.dx + c->sy / D.dy);
+            c->dt = tmin(dt, T - c->t);
+            c->active = 1;
+            return;
+        }
+        if (c->record) {
+            c->pending = c->k;
+            return;
+        }
+        c->k += 1;
+    }
+}
+
+// A frame landing is complete (its frame written): record its time and move to the next target.  hold: a
+// flux-network run, where the instance then waits for the batch's network call at this frame (unless it
+// was the last frame) and is counted in counters[kCountHeld]; otherwise it decides its next step at once.
+__device__ inline void complete_landing(const InstDesc& D, Ctl* c, int hold, int* counters) {
+    const int k = c->pending;
+    if (D.ftimes) D.ftimes[k] = c->t;
+    c->pending = -1;
+    c->k = k + 1;
+    if (hold && c->k <= c->nout) {
+        c->hold = 1;
+        atomicAdd(counters + kCountHeld, 1);
+        return;
+    }
+    decide_next(D, c, counters);
+}
+
+}  // namespace npde
