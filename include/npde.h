@@ -182,7 +182,8 @@ npde_status npde_builtin_problem(const char* test, int n, npde_grid_desc* grid, 
 
 /* ---- learned-correction network kernels (no counterpart in the reference: its networks are JAX;
  *      closest analog the periodic 3x3 convolution of surrogate/models.py:16-29) ----
- * Device pointers; activations NHWC float32 with channels a multiple of 16, values rounded to bfloat16
+ * Device pointers; activations NHWC float32 with channels a multiple of 16 (output channels, and both
+ * channel counts of the weight gradient, 16 to 64), W a multiple of 16; values rounded to bfloat16
  * at the MFMA inputs, FP32 accumulation.  wpack: bfloat16 weights in MFMA operand order
  * [tap][cin/16][cout/16][64][4].  epilogue: 0 linear (+bias), 1 +bias then tanh-GELU (pre-activation
  * to `pre` if non-NULL), 2 multiply by GELU'(gpre) (data gradient).  Return 0 on success. */
@@ -227,6 +228,23 @@ npde_status npde_get_occupancy(const npde_solver* s, int* waves_per_simd);
 npde_status npde_time_stages(npde_solver* s, int nsteps, double* ms_stage1, double* ms_stage2);
 /* Milliseconds of device time of the last npde_step / npde_advance_to / npde_run_trajectory. */
 npde_status npde_last_elapsed_ms(const npde_solver* s, double* ms);
+/* Per-kernel profile: replaces the reference's per-phase Profile (solver2d.hpp:29-52, printed by
+ * main2d.cpp:261).  While profiling is on, the handle launches its kernels one at a time on its stream
+ * (no hipGraphs) with HIP events around each launch, so the profiled run is slower, as the reference's
+ * timed run is.  Switching it on clears the accumulated times. */
+typedef enum {
+    NPDE_PROF_STAGE1 = 0,    /* stage 1: slopes, face fluxes, U1 */
+    NPDE_PROF_STAGE2 = 1,    /* stage 2: the same from U1, U^{n+1}, and the next dt's wave-speed partials */
+    NPDE_PROF_FINISH = 2,    /* per-instance dt reduction and run-state decision */
+    NPDE_PROF_FRAME = 3,     /* frame block averages and landings */
+    NPDE_PROF_WAVESPEED = 4, /* wave speeds of a fresh state (start of a run, npde_compute_dt) */
+    NPDE_PROF_NETWORK = 5,   /* flux-network evaluations */
+    NPDE_PROF_KINDS = 6
+} npde_profile_kind;
+npde_status npde_set_profiling(npde_solver* s, int on);
+/* Accumulated device milliseconds and launch counts per npde_profile_kind (arrays of NPDE_PROF_KINDS;
+ * either may be NULL). */
+npde_status npde_get_profile(npde_solver* s, double* ms, long long* launches);
 /* Stream the handle launches on (hipStream_t), for callers that order their own work after it. */
 void* npde_stream(npde_solver* s);
 

@@ -218,12 +218,13 @@ int npde_conv3x3(const float* x, int B, int H, int W, int cin, const void* wpack
     const dim3 grid((waves + 3) / 4), block(256);
     hipStream_t s = (hipStream_t)stream;
     const v4s* wp = (const v4s*)wpack;
-    if (cout == 16)
-        hipLaunchKernelGGL(k_conv_fwd<1>, grid, block, 0, s, x, B, H, W, cin, wp, bias, cout_store, y, pre, epilogue, gpre);
-    else if (cout == 32)
-        hipLaunchKernelGGL(k_conv_fwd<2>, grid, block, 0, s, x, B, H, W, cin, wp, bias, cout_store, y, pre, epilogue, gpre);
-    else
-        return 1;
+    switch (cout) {
+        case 16: hipLaunchKernelGGL(k_conv_fwd<1>, grid, block, 0, s, x, B, H, W, cin, wp, bias, cout_store, y, pre, epilogue, gpre); break;
+        case 32: hipLaunchKernelGGL(k_conv_fwd<2>, grid, block, 0, s, x, B, H, W, cin, wp, bias, cout_store, y, pre, epilogue, gpre); break;
+        case 48: hipLaunchKernelGGL(k_conv_fwd<3>, grid, block, 0, s, x, B, H, W, cin, wp, bias, cout_store, y, pre, epilogue, gpre); break;
+        case 64: hipLaunchKernelGGL(k_conv_fwd<4>, grid, block, 0, s, x, B, H, W, cin, wp, bias, cout_store, y, pre, epilogue, gpre); break;
+        default: return 1;
+    }
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
@@ -235,16 +236,14 @@ int npde_conv3x3_wgrad(const float* x, const float* g, int B, int H, int W, int 
     const int nbands = B * (H / rows);
     const dim3 grid((nbands * 9 + 3) / 4), block(256);
     hipStream_t s = (hipStream_t)stream;
-    if (cin == 16 && cout == 16)
-        hipLaunchKernelGGL((k_conv_wgrad<1, 1>), grid, block, 0, s, x, g, B, H, W, rows, part, bpart);
-    else if (cin == 16 && cout == 32)
-        hipLaunchKernelGGL((k_conv_wgrad<1, 2>), grid, block, 0, s, x, g, B, H, W, rows, part, bpart);
-    else if (cin == 32 && cout == 16)
-        hipLaunchKernelGGL((k_conv_wgrad<2, 1>), grid, block, 0, s, x, g, B, H, W, rows, part, bpart);
-    else if (cin == 32 && cout == 32)
-        hipLaunchKernelGGL((k_conv_wgrad<2, 2>), grid, block, 0, s, x, g, B, H, W, rows, part, bpart);
-    else
-        return 1;
+    if (cin <= 0 || cout <= 0 || (cin % 16) || (cout % 16) || cin > 64 || cout > 64) return 1;
+    using WgradFn = void (*)(const float*, const float*, int, int, int, int, float*, float*);
+    static constexpr WgradFn table[4][4] = {
+        {k_conv_wgrad<1, 1>, k_conv_wgrad<1, 2>, k_conv_wgrad<1, 3>, k_conv_wgrad<1, 4>},
+        {k_conv_wgrad<2, 1>, k_conv_wgrad<2, 2>, k_conv_wgrad<2, 3>, k_conv_wgrad<2, 4>},
+        {k_conv_wgrad<3, 1>, k_conv_wgrad<3, 2>, k_conv_wgrad<3, 3>, k_conv_wgrad<3, 4>},
+        {k_conv_wgrad<4, 1>, k_conv_wgrad<4, 2>, k_conv_wgrad<4, 3>, k_conv_wgrad<4, 4>}};
+    hipLaunchKernelGGL(table[cin / 16 - 1][cout / 16 - 1], grid, block, 0, s, x, g, B, H, W, rows, part, bpart);
     const int nw = 9 * cin * cout;
     hipLaunchKernelGGL(k_wgrad_reduce, dim3((nw + cout + 255) / 256), dim3(256), 0, s, part, bpart, nbands, nw, cout,
                        dw, db);

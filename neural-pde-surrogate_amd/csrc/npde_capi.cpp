@@ -81,6 +81,12 @@ struct npde_solver {
     bool fast = false;  // NPDE_ARITH_FAST: the fast-arithmetic stage kernels
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // per-kernel profile (npde_set_profiling): an event pair around each launch, read after the stream syncs
+    bool prof_on = false;
+    std::vector<hipEvent_t> prof_events;  // pool: pairs (start, end) of the recorded launches
+    std::vector<int> prof_kind;           // kind of each recorded launch not yet read
+    double prof_ms[NPDE_PROF_KINDS] = {};
+    long long prof_n[NPDE_PROF_KINDS] = {};
     npde_tuning tune{0, 4, 32, 1, 1, 1, 0};
     int lds_per_cu = 65536;    // LDS bytes per CU of the handle's device (queried at creation)
     int layout = kStrips;  // StageMode of the current task list
@@ -441,15 +447,63 @@ unsigned lds_reserve_for(const npde_solver* s, const npde_tuning& t, int row_wav
 }
 unsigned lds_reserve(const npde_solver* s) { return lds_reserve_for(s, s->tune, s->row_waves); }
 
+// Profiling: read the recorded launches' event pairs into the per-kind totals.
+npde_status prof_harvest(npde_solver* s) {
+    const size_t n = s->prof_kind.size();
+    if (n == 0) return NPDE_OK;
+    HIPCHK(s, hipEventSynchronize(s->prof_events[2 * n - 1]));
+    for (size_t i = 0; i < n; ++i) {
+        float ms = 0.f;
+        HIPCHK(s, hipEventElapsedTime(&ms, s->prof_events[2 * i], s->prof_events[2 * i + 1]));
+        s->prof_ms[s->prof_kind[i]] += ms;
+        s->prof_n[s->prof_kind[i]] += 1;
+    }
+    s->prof_kind.clear();
+    return NPDE_OK;
+}
+
+// A kernel launch (launch() returns hipError_t), between two events when profiling is on.  Profiled runs
+// launch on the stream, never under graph capture.
+constexpr size_t kProfBatch = 4096;  // recorded launches before the host reads them
+template <class F>
+npde_status timed_launch(npde_solver* s, int kind, F&& launch) {
+    if (!s->prof_on) {
+        HIPCHK(s, launch());
+        return NPDE_OK;
+    }
+    if (s->prof_kind.size() >= kProfBatch) {
+        npde_status st = prof_harvest(s);
+        if (st != NPDE_OK) return st;
+    }
+    const size_t i = s->prof_kind.size();
+    while (s->prof_events.size() < 2 * i + 2) {
+        hipEvent_t e = nullptr;
+        HIPCHK(s, hipEventCreate(&e));
+        s->prof_events.push_back(e);
+    }
+    HIPCHK(s, hipEventRecord(s->prof_events[2 * i], s->stream));
+    HIPCHK(s, launch());
+    HIPCHK(s, hipEventRecord(s->prof_events[2 * i + 1], s->stream));
+    s->prof_kind.push_back(kind);
+    return NPDE_OK;
+}
+
 // One stage launch.  In seam mode the wavefronts' seam solves read the stage input's edge columns from
 // the edge arrays, except in stage 1 of the first step of a launch sequence (first_step: a host call, a
 // graph, an iteration loop), whose input the caller may have changed since a stage 2 last wrote them.
 npde_status launch_stage_kernel(npde_solver* s, int stage, bool first_step) {
     const int compact = (stage == 2 || !first_step) ? 1 : 0;
-    HIPCHK(s, launch_stage(s->flux, s->lim, stage, s->has_corr, s->fast, s->tune.waves_per_block, s->layout,
-                           s->row_waves, compact, s->band, lds_reserve(s), s->ddesc, s->dctl, s->dtasks,
-                           int(s->htasks.size()), s->dpart, s->dndone, s->stream));
-    return NPDE_OK;
+    return timed_launch(s, stage == 1 ? NPDE_PROF_STAGE1 : NPDE_PROF_STAGE2, [&] {
+        return launch_stage(s->flux, s->lim, stage, s->has_corr, s->fast, s->tune.waves_per_block, s->layout,
+                            s->row_waves, compact, s->band, lds_reserve(s), s->ddesc, s->dctl, s->dtasks,
+                            int(s->htasks.size()), s->dpart, s->dndone, s->stream);
+    });
+}
+
+npde_status launch_finish_kernel(npde_solver* s, int mode) {
+    return timed_launch(s, NPDE_PROF_FINISH, [&] {
+        return launch_finish(s->ddesc, s->dctl, s->dpart, s->n, mode, s->dndone, s->stream);
+    });
 }
 
 // One RK2 step of every stepping instance: two stage launches, then the per-instance reduction of the
@@ -459,15 +513,16 @@ npde_status launch_step_pair(npde_solver* s, bool first_step) {
         npde_status st = launch_stage_kernel(s, stage, first_step);
         if (st != NPDE_OK) return st;
     }
-    HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterStep, s->dndone, s->stream));
-    return NPDE_OK;
+    return launch_finish_kernel(s, kAfterStep);
 }
 
 npde_status launch_wavespeed_and_finish(npde_solver* s) {
-    HIPCHK(s, launch_wavespeed(s->tune.waves_per_block, s->layout, s->row_waves, s->ddesc, s->dctl, s->dtasks,
-                               int(s->htasks.size()), s->dpart, s->dndone, s->stream));
-    HIPCHK(s, launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterWavespeed, s->dndone, s->stream));
-    return NPDE_OK;
+    npde_status st = timed_launch(s, NPDE_PROF_WAVESPEED, [&] {
+        return launch_wavespeed(s->tune.waves_per_block, s->layout, s->row_waves, s->ddesc, s->dctl, s->dtasks,
+                                int(s->htasks.size()), s->dpart, s->dndone, s->stream);
+    });
+    if (st != NPDE_OK) return st;
+    return launch_finish_kernel(s, kAfterWavespeed);
 }
 
 npde_status launch_iterations(npde_solver* s, int iters, int nft) {
@@ -475,13 +530,18 @@ npde_status launch_iterations(npde_solver* s, int iters, int nft) {
         if (nft > 0) {
             // the frame kernel writes the frames of the instances that landed and completes their landings
             const int hold = s->net_frags ? 1 : 0;
-            HIPCHK(s, launch_frame(s->ddesc, s->dctl, s->dftasks, nft, hold, s->dndone, s->stream));
+            npde_status st = timed_launch(s, NPDE_PROF_FRAME, [&] {
+                return launch_frame(s->ddesc, s->dctl, s->dftasks, nft, hold, s->dndone, s->stream);
+            });
             // flux-network run: an instance that reached its frame holds there; once every running instance
             // holds, the network turns their states into the next interval's factors (one launch for the
             // batch), and its last workgroup lets them all move on
-            if (hold)
-                HIPCHK(s, launch_flux_net(s->ddesc, s->dctl, s->dnettasks, s->nnettasks, s->n, s->dndone, s->net_frags,
-                                          s->net_bias, s->net_scale, s->dnetpeak, s->stream));
+            if (st == NPDE_OK && hold)
+                st = timed_launch(s, NPDE_PROF_NETWORK, [&] {
+                    return launch_flux_net(s->ddesc, s->dctl, s->dnettasks, s->nnettasks, s->n, s->dndone,
+                                           s->net_frags, s->net_bias, s->net_scale, s->dnetpeak, s->stream);
+                });
+            if (st != NPDE_OK) return st;
         }
         npde_status st = launch_step_pair(s, it == 0);
         if (st != NPDE_OK) return st;
@@ -562,7 +622,8 @@ npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, boo
     long long expect = expected_steps(s, false, t_end, max_new_steps);
     int level = loop_level(s, expect);
     bool frames = nft > 0;  // the frame kernels in this launch
-    if (s->tune.use_graph && (st = ensure_graph(s, nft, level, frames)) != NPDE_OK) return st;
+    const bool graphs = s->tune.use_graph && !s->prof_on;  // a profiled run launches on the stream
+    if (graphs && (st = ensure_graph(s, nft, level, frames)) != NPDE_OK) return st;
     HIPCHK(s, launch_begin_run(s->dctl, s->n, t_start, t_end, nout, record ? 1 : 0, max_new_steps, s->dndone,
                                s->stream));
     HIPCHK(s, hipEventRecord(s->ev0, s->stream));
@@ -574,7 +635,7 @@ npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, boo
     long long iters = 0;
     for (;;) {
         if (iters >= 50000000LL) return fail(s, NPDE_ERR_STATE, "run did not finish within 5e7 steps");
-        if (s->tune.use_graph) {
+        if (graphs) {
             if ((st = ensure_graph(s, nft, level, frames)) != NPDE_OK) return st;
             HIPCHK(s, hipGraphLaunch(s->graphs[frames ? 1 : 0][level], s->stream));
         } else if ((st = launch_iterations(s, loop_iters(s, level), frames ? nft : 0)) != NPDE_OK) {
@@ -584,6 +645,7 @@ npde_status auto_run(npde_solver* s, double t_start, double t_end, int nout, boo
         HIPCHK(s, hipMemcpyAsync(&done, s->dndone, sizeof(int), hipMemcpyDeviceToHost, s->stream));
         HIPCHK(s, hipMemcpyAsync(s->hctl.data(), s->dctl, sizeof(Ctl) * s->n, hipMemcpyDeviceToHost, s->stream));
         HIPCHK(s, hipStreamSynchronize(s->stream));
+        if ((st = prof_harvest(s)) != NPDE_OK) return st;
         if (done >= s->n) break;
         if (2 * (s->n - done) <= listed_n && listed_n > 1 && (st = retile_tail(s, listed_n)) != NPDE_OK)
             return st;
@@ -733,6 +795,7 @@ npde_status npde_destroy(npde_solver* s) {
                     (void*)s->dftasks, (void*)s->dpart, (void*)s->dndone, (void*)s->ddts, (void*)s->dstate,
                     (void*)s->dtraj, (void*)s->dftimes})
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : s->prof_events) (void)hipEventDestroy(e);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -933,8 +996,7 @@ npde_status npde_time_stages(npde_solver* s, int nsteps, double* ms_stage1, doub
             if (he == hipSuccess) st = launch_stage_kernel(s, stage, k == 0);
         }
         if (st == NPDE_OK && he == hipSuccess) he = hipEventRecord(e[2], s->stream);
-        if (st == NPDE_OK && he == hipSuccess)
-            he = launch_finish(s->ddesc, s->dctl, s->dpart, s->n, kAfterStep, s->dndone, s->stream);
+        if (st == NPDE_OK && he == hipSuccess) st = launch_finish_kernel(s, kAfterStep);
         if (st == NPDE_OK && he == hipSuccess) he = hipEventSynchronize(e[2]);
         for (int q = 0; q < 2 && st == NPDE_OK && he == hipSuccess; ++q) {
             float m = 0.f;
@@ -1161,8 +1223,11 @@ npde_status npde_set_flux_network(npde_solver* s, const void* weights, const flo
 npde_status npde_flux_network_eval(npde_solver* s) {
     if (!s) return NPDE_ERR_INVALID;
     if (!s->net_frags) return fail(s, NPDE_ERR_STATE, "no flux network set");
-    HIPCHK(s, launch_flux_net(s->ddesc, s->dctl, s->dnettasks, s->nnettasks, s->n, nullptr, s->net_frags, s->net_bias,
-                              s->net_scale, s->dnetpeak, s->stream));
+    npde_status st = timed_launch(s, NPDE_PROF_NETWORK, [&] {
+        return launch_flux_net(s->ddesc, s->dctl, s->dnettasks, s->nnettasks, s->n, nullptr, s->net_frags,
+                               s->net_bias, s->net_scale, s->dnetpeak, s->stream);
+    });
+    if (st != NPDE_OK) return st;
     HIPCHK(s, hipStreamSynchronize(s->stream));
     return NPDE_OK;
 }
@@ -1229,6 +1294,30 @@ npde_status npde_get_task_layout(const npde_solver* s, int* layout) {
                       int(NPDE_TASKS_SEAMS) == kSeams,
                   "npde_task_layout mirrors StageMode");
     *layout = s->layout;
+    return NPDE_OK;
+}
+
+npde_status npde_set_profiling(npde_solver* s, int on) {
+    if (!s) return NPDE_ERR_INVALID;
+    npde_status st = prof_harvest(s);  // launches recorded before the switch
+    if (st != NPDE_OK) return st;
+    s->prof_on = on != 0;
+    if (s->prof_on)
+        for (int k = 0; k < NPDE_PROF_KINDS; ++k) {
+            s->prof_ms[k] = 0.0;
+            s->prof_n[k] = 0;
+        }
+    return NPDE_OK;
+}
+
+npde_status npde_get_profile(npde_solver* s, double* ms, long long* launches) {
+    if (!s) return NPDE_ERR_INVALID;
+    npde_status st = prof_harvest(s);
+    if (st != NPDE_OK) return st;
+    for (int k = 0; k < NPDE_PROF_KINDS; ++k) {
+        if (ms) ms[k] = s->prof_ms[k];
+        if (launches) launches[k] = s->prof_n[k];
+    }
     return NPDE_OK;
 }
 

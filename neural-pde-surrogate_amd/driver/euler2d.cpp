@@ -182,6 +182,8 @@ int main(int argc, char* argv[]) {
     }
     npde_bad_cell bad{};
     check(s, npde_set_state(s, 0, U.data(), NPDE_LAYOUT_PLANES), bad);
+    const bool profile = a["profile"] == "yes";
+    if (profile) check(s, npde_set_profiling(s, 1), bad);
 
     double t = 0.0;
     long long steps = 0;
@@ -245,15 +247,33 @@ int main(int argc, char* argv[]) {
     const double mcells = double(g.nx) * g.ny * steps / secs / 1e6;
     std::printf("Done: %lld steps in %.2f s (%.2f million cell-updates per second). Output in %s\n", steps, secs,
                 mcells, traj_mode ? a["traj"].c_str() : (dir + "/").c_str());
-    if (a["profile"] == "yes") {
+    if (profile) {
+        // the reference's per-phase table (solver2d.hpp:43-51): here one row per kernel, from HIP events
+        // around every launch of the (unbatched, graph-free) profiled run, then the launch gaps inside the
+        // device loop, the output and the host's own time
+        double kms[NPDE_PROF_KINDS] = {};
+        long long kn[NPDE_PROF_KINDS] = {};
+        check(s, npde_get_profile(s, kms, kn), bad);
+        const char* names[NPDE_PROF_KINDS] = {"stage 1 (slopes, fluxes, U1)", "stage 2 (+ U^n+1, wave speeds)",
+                                               "dt reduction + run state", "frames (block averages)",
+                                               "wave speeds (fresh state)", "flux network"};
         const double cells = double(g.nx) * g.ny * double(steps > 0 ? steps : 1);
-        const double dev = device_ms * 1e-3, other = secs - dev - output_s;
-        std::printf("\n%-30s %9s %7s %12s\n", "phase", "time (s)", "share", "ns per cell");
-        const char* names[3] = {"device loop (2 stages + dt)", "output (copy + HDF5)", "host (setup, sync)"};
-        const double tt[3] = {dev, output_s, other > 0 ? other : 0.0};
-        for (int k = 0; k < 3; ++k)
-            std::printf("%-30s %9.3f %6.1f%% %12.2f\n", names[k], tt[k], 100.0 * tt[k] / secs, 1e9 * tt[k] / cells);
-        std::printf("%-30s %9.3f %6.1f%% %12.2f\n", "total", secs, 100.0, 1e9 * secs / cells);
+        const double dev = device_ms * 1e-3;
+        double kern = 0.0;
+        for (double v : kms) kern += v * 1e-3;
+        const double gaps = dev > kern ? dev - kern : 0.0, other = secs - dev - output_s;
+        auto row = [&](const char* name, double t, long long n) {
+            std::printf("%-32s %9.3f %6.1f%% %12.2f", name, t, 100.0 * t / secs, 1e9 * t / cells);
+            if (n >= 0) std::printf(" %9lld", n);
+            std::printf("\n");
+        };
+        std::printf("\n%-32s %9s %7s %12s %9s\n", "phase", "time (s)", "share", "ns per cell", "launches");
+        for (int k = 0; k < NPDE_PROF_KINDS; ++k)
+            if (kn[k] > 0) row(names[k], kms[k] * 1e-3, kn[k]);
+        row("launch gaps (device loop)", gaps, -1);
+        row("output (copy + HDF5)", output_s, -1);
+        row("host (setup, sync)", other > 0 ? other : 0.0, -1);
+        row("total", secs, -1);
     }
     return 0;
 }

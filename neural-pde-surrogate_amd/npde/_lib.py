@@ -71,6 +71,8 @@ _SIGNATURES = [
     ("npde_get_arithmetic", C.c_int, [_H, C.POINTER(C.c_int)]),
     ("npde_get_task_layout", C.c_int, [_H, C.POINTER(C.c_int)]),
     ("npde_get_occupancy", C.c_int, [_H, C.POINTER(C.c_int)]),
+    ("npde_set_profiling", C.c_int, [_H, C.c_int]),
+    ("npde_get_profile", C.c_int, [_H, _D, C.POINTER(C.c_longlong)]),
     ("npde_last_elapsed_ms", C.c_int, [_H, _D]),
     ("npde_stream", C.c_void_p, [_H]),
     ("npde_builtin_problem", C.c_int, [C.c_char_p, C.c_int, C.POINTER(GridDesc), _D, _D]),

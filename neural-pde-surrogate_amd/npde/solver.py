@@ -381,6 +381,20 @@ class Batch:
         check(self._h, self._lib.npde_autotune(self._h, cache_path.encode() if cache_path else None, C.byref(t)))
         return t.as_dict()
 
+    PROFILE_KINDS = ("stage1", "stage2", "finish", "frame", "wavespeed", "network")
+
+    def set_profiling(self, on: bool = True):
+        """Per-kernel profile: while on, kernels launch one at a time (no graphs) between HIP events (the profiled
+        run is slower).  Switching on clears the accumulated times."""
+        check(self._h, self._lib.npde_set_profiling(self._h, int(bool(on))))
+
+    def profile(self) -> dict:
+        """{kind: (device milliseconds, launches)} accumulated while profiling was on."""
+        ms = (C.c_double * len(self.PROFILE_KINDS))()
+        n = (C.c_longlong * len(self.PROFILE_KINDS))()
+        check(self._h, self._lib.npde_get_profile(self._h, ms, n))
+        return {k: (ms[i], n[i]) for i, k in enumerate(self.PROFILE_KINDS)}
+
     def last_elapsed_ms(self) -> float:
         ms = C.c_double()
         check(self._h, self._lib.npde_last_elapsed_ms(self._h, C.byref(ms)))

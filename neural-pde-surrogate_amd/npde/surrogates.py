@@ -13,7 +13,11 @@ The architectures of the reference's surrogate/models.py (Equinox), rebuilt on t
                                are positive by construction.
 
 All convolutions use periodic padding, so the ResNet and FNO are exactly translation-equivariant on the
-torus (the U-Net for shifts that are multiples of 2**levels).  GELU is the tanh approximation (JAX's
+torus (the U-Net for shifts that are multiples of 2**levels).  ResNet(backend="mfma") runs its k = 3
+convolutions on the library's BF16-MFMA periodic convolution kernels (npde_conv3x3 and its weight
+gradient; forward and backward) in NHWC, with the GELU that follows the first convolution of each block in
+the kernel's epilogue; the 1x1 lift and projection are matrix products.  Its parameters are the torch
+backend's, so a model can switch backends.  GELU is the tanh approximation (JAX's
 default, which the reference uses).  Parameter shapes follow the reference, so parameter counts agree:
 ResNet(64, 8) 591,428; UNet(16, 3) 516,516; FNO(32, 4, 16 modes) 4,203,428.
 """
@@ -72,20 +76,77 @@ class ResBlock(torch.nn.Module):
         return x + self.conv2(gelu(self.conv1(gelu(x))))
 
 
+class Conv3x3MFMA(torch.autograd.Function):
+    """One 3x3 periodic convolution on the library's BF16-MFMA kernels, NHWC float32 (B, H, W, C) with C a
+    multiple of 16 (at most 64), optionally followed by tanh-GELU in the kernel's epilogue.  Backward: the
+    pre-activation gradient (GELU' of the saved pre-activation), the weight and bias gradients
+    (npde_conv3x3_wgrad, fixed-order band reduction) and the data gradient (the transposed, tap-reversed
+    convolution)."""
+
+    @staticmethod
+    def forward(ctx, x, weight, bias, apply_gelu):
+        from .learned import conv as K
+        cout = weight.shape[0]
+        y, pre = K.conv3x3(x.contiguous(), K.pack_weights(weight.detach()), bias.detach().contiguous(), cout, cout,
+                           K.EPI_GELU if apply_gelu else K.EPI_LINEAR, want_pre=apply_gelu)
+        ctx.save_for_backward(x, weight, pre)
+        ctx.apply_gelu = apply_gelu
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        from .learned import conv as K
+        x, weight, pre = ctx.saved_tensors
+        g = gy.contiguous()
+        if ctx.apply_gelu:
+            with torch.enable_grad():
+                z = pre.detach().requires_grad_(True)
+                g, = torch.autograd.grad(gelu(z), z, g)
+        dw, db = K.conv3x3_wgrad(x.contiguous(), g.contiguous())
+        cin = x.shape[-1]
+        gx, _ = K.conv3x3(g.contiguous(), K.pack_weights(weight.detach(), transpose_flip=True), None, cin, cin,
+                          K.EPI_LINEAR)
+        return gx, dw.contiguous(), db, None
+
+
+def _conv1x1_nhwc(h, conv: "PeriodicConv"):
+    """A 1x1 PeriodicConv applied to NHWC activations: a matrix product over the channel axis."""
+    return torch.matmul(h, conv.weight[:, :, 0, 0].t()) + conv.bias
+
+
 class ResNet(torch.nn.Module):
-    def __init__(self, width=64, depth=8, kernel_size=3, delta_scale=None, channels=4, seed=0):
+    def __init__(self, width=64, depth=8, kernel_size=3, delta_scale=None, channels=4, seed=0, backend="torch"):
         super().__init__()
         g = torch.Generator().manual_seed(seed)
         self.lift = PeriodicConv(channels, width, 1, g)
         self.blocks = torch.nn.ModuleList([ResBlock(width, kernel_size, g) for _ in range(depth)])
         self.project = PeriodicConv(width, channels, 1, g).zero_()
         self.register_buffer("delta_scale", _scale(delta_scale, channels))
+        self.kernel_size = kernel_size
+        self.backend = backend
 
     def forward(self, x):
+        if self.backend == "mfma":
+            return self._forward_mfma(x)
         h = self.lift(x)
         for b in self.blocks:
             h = b(h)
         return x + self.delta_scale.view(1, -1, 1, 1).to(x.dtype) * self.project(gelu(h))
+
+    def _forward_mfma(self, x):
+        """The same network in NHWC with the 3x3 convolutions on the MFMA kernels (float32 activations,
+        BF16 products, FP32 accumulation)."""
+        width = self.lift.weight.shape[0]
+        H, W = x.shape[-2:]
+        if self.kernel_size != 3 or width % 16 or width > 64 or W % 16 or H % 8 or x.dtype != torch.float32:
+            raise ValueError("the MFMA backend needs k = 3, width a multiple of 16 up to 64, W a multiple of 16, "
+                             "H a multiple of 8 and float32 input")
+        h = _conv1x1_nhwc(x.permute(0, 2, 3, 1), self.lift)
+        for b in self.blocks:
+            t = Conv3x3MFMA.apply(gelu(h), b.conv1.weight, b.conv1.bias, True)
+            h = h + Conv3x3MFMA.apply(t, b.conv2.weight, b.conv2.bias, False)
+        out = _conv1x1_nhwc(gelu(h), self.project).permute(0, 3, 1, 2)
+        return x + self.delta_scale.view(1, -1, 1, 1).to(x.dtype) * out
 
 
 class ConvBlock(torch.nn.Module):

@@ -65,6 +65,63 @@ def test_trajectory_file_matches_reference_cli(npde_mod, tmp_path):
         assert np.array_equal(h5(out, f"U_{m}", "<f4").reshape(5, 4, m, m), t[f"U_{m}"]), m
 
 
+@pytest.mark.gpu
+def test_profile_table_splits_the_run_by_kernel(npde_mod, tmp_path):
+    """--profile yes prints the reference's phase table (solver2d.hpp:43-51) with one row per kernel kind
+    (launch counts from events around every launch of the graph-free profiled run); the profiled run's
+    trajectory is bitwise the unprofiled one."""
+    t = np.load(GOLDEN / "traj_ll64.npz")
+    out = tmp_path / "ll64.h5"
+    p = run("--test", "lax_liu3", "--n", "64", "--nout", "4", "--traj", out, "--save", "64,32,16",
+            "--profile", "yes")
+    assert p.returncode == 0, p.stderr
+    steps = int(re.search(r"Done: (\d+) steps", p.stdout).group(1))
+    rows = {m.group(1).strip(): (float(m.group(2)), int(m.group(3)) if m.group(3) else None)
+            for m in re.finditer(r"^(\S.*?)\s+([0-9.]+)\s+[0-9.]+%\s+[0-9.]+(?:\s+(\d+))?$", p.stdout, re.M)}
+    assert rows["stage 1 (slopes, fluxes, U1)"][1] == rows["stage 2 (+ U^n+1, wave speeds)"][1] >= steps
+    assert rows["dt reduction + run state"][1] > rows["stage 2 (+ U^n+1, wave speeds)"][1]
+    assert rows["frames (block averages)"][1] >= 5
+    assert rows["stage 1 (slopes, fluxes, U1)"][0] > 0 and rows["stage 2 (+ U^n+1, wave speeds)"][0] > 0
+    for k in ("launch gaps (device loop)", "output (copy + HDF5)", "host (setup, sync)", "total"):
+        assert k in rows
+    assert np.array_equal(h5(out, "t", "<f8"), t["t"])
+    for m in (64, 32, 16):
+        assert np.array_equal(h5(out, f"U_{m}", "<f4").reshape(5, 4, m, m), t[f"U_{m}"]), m
+
+
+@pytest.mark.gpu
+def test_profiled_batch_matches_the_unprofiled_run(npde_mod):
+    """npde_set_profiling: a profiled device-resident run (kernels launched one at a time between events) gives
+    the same bits, step counts and frames as the graph-launched run; the profile counts one stage-1 and one
+    stage-2 launch per loop iteration and at least one per step, and switching it on again clears it."""
+    from npde.families import riemann
+    n, B = 128, 3
+    ics = [riemann(40 + k, n)[0] for k in range(B)]
+    grid = npde_mod.make_grid(n, n, bc_x=1, bc_y=1)
+    res = []
+    for prof in (False, True):
+        with npde_mod.Batch([grid] * B, "hllc", "vanleer", 0.5) as b:
+            b.set_profiling(prof)
+            for k, U in enumerate(ics):
+                b.set_state(k, U)
+            b.run_trajectory(0.05, 5, sizes=(64,))
+            res.append(([b.get_state(k) for k in range(B)], [b.trajectory(k, 64) for k in range(B)], b.times()[1]))
+            pr = b.profile()
+            if prof:
+                assert pr["stage1"][1] == pr["stage2"][1] >= res[-1][2].max()
+                assert pr["finish"][1] >= pr["stage2"][1] and pr["frame"][1] >= 6 and pr["wavespeed"][1] >= 1
+                assert pr["stage1"][0] > 0 and pr["stage2"][0] > 0 and pr["network"] == (0.0, 0)
+                b.set_profiling(True)
+                assert all(v == (0.0, 0) for v in b.profile().values())
+            else:
+                assert all(v == (0.0, 0) for v in pr.values())
+    (sa, fa, na), (sb, fb, nb) = res
+    assert (na == nb).all()
+    for k in range(B):
+        assert np.array_equal(sa[k].view(np.uint64), sb[k].view(np.uint64))
+        assert np.array_equal(fa[k], fb[k])
+
+
 def layout(path):
     """`h5dump -p -A` of a file (names, types, shapes, storage, filters, attribute values; no dataset
     data) with its path replaced by its base name, as tests/golden/make_golden.py records it."""

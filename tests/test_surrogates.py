@@ -187,6 +187,47 @@ def test_resnet_learns_solver_frames_on_gpu(npde_mod):
     assert err < 0.8 * persist, (err, persist)
 
 
+@pytest.mark.gpu
+def test_resnet_on_the_mfma_kernels_matches_the_torch_backend(npde_mod):
+    """ResNet(backend="mfma") -- its 3x3 convolutions on the library's BF16-MFMA kernels, forward and backward --
+    against the same parameters on torch's float32 convolutions: output change and every parameter gradient
+    within BF16 tolerance (relative L2 error below 3e-2 and 5e-2), for widths 32 and 64; and a few Adam steps
+    on the MFMA backend lower the one-batch loss."""
+    dev = torch.device("cuda")
+    for width in (32, 64):
+        m = M.ResNet(width=width, depth=2, seed=3).to(dev)
+        with torch.no_grad():  # a non-zero projection, so that the output depends on the trunk
+            g = torch.Generator().manual_seed(1)
+            m.project.weight.copy_(torch.randn(m.project.weight.shape, generator=g) * 0.1)
+            m.project.bias.copy_(torch.randn(m.project.bias.shape, generator=g) * 0.1)
+        x = torch.randn(4, 4, 64, 64, generator=torch.Generator().manual_seed(2)).to(dev)
+        out, grads = {}, {}
+        for backend in ("torch", "mfma"):
+            m.backend = backend
+            m.zero_grad()
+            y = m(x)
+            (y ** 2).mean().backward()
+            out[backend] = (y - x).detach()
+            grads[backend] = {k: p.grad.detach().clone() for k, p in m.named_parameters()}
+        rel = lambda a, b: float((a - b).norm() / b.norm())
+        assert rel(out["mfma"], out["torch"]) < 3e-2, width
+        for k in grads["torch"]:
+            assert rel(grads["mfma"][k], grads["torch"][k]) < 5e-2, (width, k)
+    m.backend = "mfma"
+    y_t = (x + 0.1 * torch.randn(x.shape, generator=torch.Generator().manual_seed(4)).to(dev)).detach()
+    opt = torch.optim.Adam(m.parameters(), lr=1e-3)
+    losses = []
+    for _ in range(8):
+        opt.zero_grad()
+        loss = ((m(x) - y_t) ** 2).mean()
+        loss.backward()
+        opt.step()
+        losses.append(float(loss))
+    assert losses[-1] < 0.9 * losses[0], losses
+    with pytest.raises(ValueError):
+        M.ResNet(width=128, depth=1, backend="mfma").to(dev)(x)
+
+
 def test_rollout_drift_and_bad_cells_hand_computed():
     """surrogate/rollout.py:54-62: drift of each total from the truth's first frame over its total absolute
     amount, and the count of cells with rho <= 0 or p not > 0, on frames small enough to compute by hand."""
