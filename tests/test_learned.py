@@ -328,7 +328,7 @@ def test_trained_corrector_reduces_coarse_error(npde_mod, cuda):
     assert own.sum() >= len(idx) - 2, fell
     e_plain = P.l1_density(plain[own], truth[64][own])[:, 1:].mean()
     e_learned = P.l1_density(learned[own], truth[64][own])[:, 1:].mean()
-    assert e_learned < 0.95 * e_plain, (e_learned, e_plain)
+    assert e_learned < 0.8 * e_plain, (e_learned, e_plain)
 
 
 @pytest.mark.gpu
