@@ -464,6 +464,9 @@ def main(argv=None):
                     "(a new corrector, or a change for the one given by --init)")
     ap.add_argument("--kh-eval", type=int, default=0, help="only evaluate the trained weights on this many held-out "
                     "Kelvin-Helmholtz instances at 128^2 (kh128_eval.json)")
+    ap.add_argument("--rollout", choices=("plain", "learned"), default=None, help="only time the 64^2 rollout of "
+                    "the held-out instances (--test-traj of them, in one batch) three times, plain or learned "
+                    "(rollout_times.json): a workload for a kernel trace of either loop alone")
     args = ap.parse_args(argv)
     out = Path(args.out)
     out.mkdir(parents=True, exist_ok=True)
@@ -476,6 +479,23 @@ def main(argv=None):
         logf.flush()
 
     torch.cuda.set_device(0)
+    if args.rollout:
+        idx = list(range(100000, 100000 + args.test_traj))
+        U0 = np.stack([block_average_f64(riemann(i, FINE)[0], COARSE) for i in idx])
+        grids = [S.make_grid(COARSE, COARSE, bc_x=S.PERIODIC, bc_y=S.PERIODIC)] * len(idx)
+        rows = []
+        with S.Batch(grids, "hllc", "vanleer", 0.5) as b:
+            if args.rollout == "learned":
+                from .model import load_trained
+                b.set_flux_network(load_trained(args.init) if args.init else load_trained())
+            for rep in range(3):
+                secs = _timed_run(b, list(U0), T_END, NOUT, (COARSE,), net=args.rollout == "learned")
+                steps = b.times()[1]
+                rows.append({"seconds": secs, "steps_mean": float(steps.mean()), "steps_max": int(steps.max())})
+                log(f"{args.rollout} {COARSE}^2 rollout of {len(idx)} instances: {secs * 1e3:.2f} ms, steps mean "
+                    f"{steps.mean():.1f}, max {steps.max()}")
+        (out / "rollout_times.json").write_text(json.dumps({"rollout": args.rollout, "runs": rows}, indent=1))
+        return
     if args.kh_eval:
         from .model import load_trained
         log(f"held-out Kelvin-Helmholtz evaluation: {args.kh_eval} instances")
