@@ -271,6 +271,7 @@ def learned_eval():
     return {"metric": "time-to-matched-L1 speed-up, learned 64^2 vs plain", "value": m["speedup"],
             "target": 5.0, "higher_is_better": True, "learned_l1_rho": lr["l1_rho"], "learned_seconds": lr["seconds"],
             "fallbacks": lr["fallbacks"], "fallback_seconds": lr["fallback_seconds"],
+            "network_switched_off": lr["network_switched_off"],
             "matched_plain_resolution": m["plain_resolution"],
             "matched_plain_seconds": m["plain_seconds"],
             "plain": {str(n): v for n, v in res["plain"].items()},

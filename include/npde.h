@@ -158,14 +158,23 @@ npde_status npde_set_flux_correction(npde_solver* s, int instance, const double*
  * The fused inference kernel of the flux-correction network (8 features -> four 3x3 periodic convolutions
  * of 32 channels with tanh-GELU -> a 3x3 head of 8 channels -> out_scale * tanh) writes dimensionless
  * factors into correction planes the library allocates for every instance, in the layout and with the
- * meaning of npde_set_flux_correction.  npde_run_trajectory evaluates it at every frame landing, for the
- * instances that landed, from their state at that time (before their next step); npde_advance_to keeps
- * the factors of the last evaluation.  weights: device pointer to the 69 packed bfloat16 MFMA fragments of
+ * meaning of npde_set_flux_correction.  npde_run_trajectory evaluates it at every frame landing, from the
+ * instances' states at that time (before their next step): an instance that lands holds until every
+ * running instance has landed on that frame, one launch evaluates them all, and they move on together.
+ * npde_advance_to keeps the factors of the last evaluation.  weights: device pointer to the 69 packed bfloat16 MFMA fragments of
  * the five layers (64 lanes x 8 values each: first layer 3 k-blocks x 2 n-tiles, three hidden layers 9 x 2,
  * head 9 x 1; npde.learned.model.FluxCorrector.pack_fused writes them); biases: device pointer to
  * 32 + 32 + 32 + 32 + 8 float32.  Both stay the caller's and must stay alive while the network is set.
  * Every instance must be periodic with nx and ny multiples of 16.  weights = NULL switches it off. */
 npde_status npde_set_flux_network(npde_solver* s, const void* weights, const float* biases, double out_scale);
+/* In npde_run_trajectory with the network set: switch the network off for an instance (its factors are
+ * zero from then on, i.e. the plain coarse solver) at the first frame landing where its CFL signal-speed
+ * sum max(|u|+a)/dx + max(|v|+a)/dy exceeds `ratio` times its value at the run's first frame -- a
+ * corrected flow draining a cell toward vacuum at finite pressure, whose shrinking steps would hold back
+ * the whole batch.  0: never.  Default 2.5. */
+npde_status npde_set_flux_network_limit(npde_solver* s, double ratio);
+/* Per instance (array of npde_num_instances), 1 if the last run switched its network off. */
+npde_status npde_flux_network_switched_off(npde_solver* s, int* off);
 /* Evaluate the network now for every instance (factors from the current states). */
 npde_status npde_flux_network_eval(npde_solver* s);
 /* The error indicator of the learned solver: per instance (host array of n), the largest density the network

@@ -233,6 +233,8 @@ __global__ void k_begin_run(Ctl* __restrict__ ctl, int n, double t_start, double
     c.error = 0;
     c.active = 0;
     c.hold = 0;
+    c.net_off = 0;
+    c.rate0 = 0.0;
 }
 
 // Arithmetic self-test, element-wise: the shared-reciprocal quotient and the plain IEEE quotient,

@@ -225,6 +225,18 @@ __global__ __launch_bounds__(kNetThreads) void k_flux_net(const InstDesc* __rest
         if (counters && !ctl[tk.inst].hold) continue;
         const InstDesc& D = desc[tk.inst];
         if (D.corr_x == nullptr || D.corr_y == nullptr) continue;
+        if (counters && ctl[tk.inst].net_off) {  // switched off at this landing or before: zero factors
+            double* cx = const_cast<double*>(D.corr_x);
+            double* cy = const_cast<double*>(D.corr_y);
+            for (int c = threadIdx.x; c < kNetTile * kNetTile; c += blockDim.x) {
+                const long long off = (long long)(tk.y0 + c / kNetTile) * D.nx + tk.x0 + c % kNetTile;
+                for (int q = 0; q < 4; ++q) {
+                    cx[off + q * D.plane] = 0.0;
+                    cy[off + q * D.plane] = 0.0;
+                }
+            }
+            continue;
+        }
         net_tile(D, tk, bufA, bufB, frags, bias, out_scale, peak);
         __syncthreads();  // the next tile's features overwrite the buffer the head read
     }

@@ -53,6 +53,8 @@ struct InstDesc {
     int lvl_f[kMaxLevels];       // coarsening factor nx / m
     float* traj[kMaxLevels];     // (frames, 4, ny/f, m) float32 per level, or null
     double* ftimes;              // time at which each frame was written (nout + 1), or null
+    double net_rate_limit;       // flux-network runs: switch the network off for the instance when its CFL
+                                 // signal-speed sum exceeds this multiple of its value at the first frame (0: never)
 };
 
 // Mutable per-instance run state.  Written by k_finish (one wavefront per instance), by the last frame
@@ -75,6 +77,8 @@ struct Ctl {
     int bad;              // row-major index of the first non-physical cell, or kNoBad
     int hold;             // flux-network runs: landed on a frame, waiting for the batch's network call
     int fdone;            // frame workgroups of the pending frame that have written their part (0 between launches)
+    int net_off;          // flux-network runs: the network is switched off for this instance (zero factors)
+    double rate0;         // flux-network runs: sx/dx + sy/dy at the run's first frame landing
 };
 
 // Device counters of a run, one int each: instances done; (flux-network runs) instances holding at a

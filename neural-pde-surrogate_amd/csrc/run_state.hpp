@@ -55,9 +55,19 @@ __device__ inline void decide_next(const InstDesc& D, Ctl* c, int* ndone) {
 // A frame landing is complete (its frame written): record its time and move to the next target.  hold: a
 // flux-network run, where the instance then waits for the batch's network call at this frame (unless it
 // was the last frame) and is counted in counters[kCountHeld]; otherwise it decides its next step at once.
+// In a flux-network run the landing also watches the instance's CFL signal-speed sum sx/dx + sy/dy: the
+// first frame records it, and a later frame at which it exceeds D.net_rate_limit times that value switches
+// the network off for the instance (its factors are zero from this frame on: the plain coarse solver).  A
+// corrected flow that drains a cell toward vacuum at finite pressure shows there first -- the sound speed,
+// and with it every step of the whole frame-synchronous batch, grows without bound.
 __device__ inline void complete_landing(const InstDesc& D, Ctl* c, int hold, int* counters) {
     const int k = c->pending;
     if (D.ftimes) D.ftimes[k] = c->t;
+    if (hold) {
+        const double rate = c->sx / D.dx + c->sy / D.dy;
+        if (k == 0) c->rate0 = rate;
+        else if (D.net_rate_limit > 0.0 && rate > D.net_rate_limit * c->rate0) c->net_off = 1;
+    }
     c->pending = -1;
     c->k = k + 1;
     if (hold && c->k <= c->nout) {

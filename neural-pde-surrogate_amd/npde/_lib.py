@@ -84,6 +84,8 @@ _SIGNATURES = [
     ("npde_mfma_probe", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("npde_mfma32_probe", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("npde_set_flux_network", C.c_int, [_H, C.c_void_p, C.c_void_p, C.c_double]),
+    ("npde_set_flux_network_limit", C.c_int, [_H, C.c_double]),
+    ("npde_flux_network_switched_off", C.c_int, [_H, _I]),
     ("npde_flux_network_eval", C.c_int, [_H]),
     ("npde_flux_network_peak", C.c_int, [_H, _F, C.c_int]),
     ("npde_flux_network_planes", C.c_int, [_H, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),

@@ -270,15 +270,17 @@ def plain_rollout(indices, n, m=COARSE, batch=64, n_fine=FINE):
     return np.concatenate(frames), secs
 
 
-def learned_rollout_states(model: FluxCorrector, U0: np.ndarray, tf, force_fallback=False):
+def learned_rollout_states(model: FluxCorrector, U0: np.ndarray, tf, force_fallback=False, info=None):
     """The learned coarse solver from the periodic states U0 (B, 4, n, n) binary64 over the frame times tf
     (tf[0] = 0: the states' own time): one device-resident recorded run with the network in the loop --
-    at every frame landing of an instance the library's fused network kernel turns its state into the
-    face factors of the next interval, and the stage kernels scale and add them (npde_set_flux_network).
-    Returns frames (B, F, 4, n, n) float32, the seconds of the run with its failure decision, and the
-    mask of instances that failed: the solver's positivity check stopped them (their time is short of
-    the end), or the network saw a density above BLOWUP x the instance's initial maximum (the indicator
-    the fused kernel keeps on the device)."""
+    at every frame landing (all instances land together) the library's fused network kernel turns their
+    states into the face factors of the next interval, and the stage kernels scale and add them
+    (npde_set_flux_network).  Returns frames (B, F, 4, n, n) float32, the seconds of the run with its
+    failure decision, and the mask of instances that failed: the solver's positivity check stopped them
+    (their time is short of the end), or the network saw a density above BLOWUP x the instance's initial
+    maximum (the indicator the fused kernel keeps on the device).  A dict passed as `info` receives under
+    "switched_off" the mask of instances whose network the library switched off during the run (their
+    signal speed ran away: they finished as the plain coarse solver; npde_set_flux_network_limit)."""
     B, n = U0.shape[0], U0.shape[-1]
     nout, t_end = len(tf) - 1, float(tf[-1])
     calls = model.calls_per_frame  # the run lands calls x nout times; every calls-th landing is a frame
@@ -295,6 +297,8 @@ def learned_rollout_states(model: FluxCorrector, U0: np.ndarray, tf, force_fallb
             failed[:] = True
         secs += time.perf_counter() - t0
         frames = np.stack([b.trajectory(k, n)[::calls] for k in range(B)])
+        if info is not None:
+            info["switched_off"] = b.flux_network_switched_off()
     return frames, secs, failed
 
 
@@ -304,14 +308,17 @@ def learned_rollout(model: FluxCorrector, indices, n=COARSE, batch=64, force_fal
     frames (N, F, 4, n, n) float32, seconds, and the indices that fell back to the plain fine solve
     (whose frames replace theirs; its time is counted).  The fallback instances of a batch are solved
     together, in one batch of the plain solver.  A dict passed as `timing` receives the seconds of the
-    fallback solves alone under "fallback"."""
+    fallback solves alone under "fallback", and the indices whose network the library switched off (they
+    keep their own frames) under "switched_off"."""
     tf = frame_times()
-    frames_all, secs, fell = [], 0.0, []
+    frames_all, secs, fell, switched = [], 0.0, [], []
     fallback_secs = 0.0
     for s0 in range(0, len(indices), batch):
         idx = list(indices[s0:s0 + batch])
         U0 = np.stack([block_average_f64(riemann(i, n_fine)[0], n) for i in idx])
-        out, sec, badn = learned_rollout_states(model, U0, tf, force_fallback)
+        info = {}
+        out, sec, badn = learned_rollout_states(model, U0, tf, force_fallback, info=info)
+        switched += [idx[k] for k in np.nonzero(info["switched_off"] & ~badn)[0]]
         secs += sec
         if badn.any():
             redo = [idx[k] for k in np.nonzero(badn)[0]]
@@ -327,6 +334,10 @@ def learned_rollout(model: FluxCorrector, indices, n=COARSE, batch=64, force_fal
         frames_all.append(out)
     if timing is not None:
         timing["fallback"] = fallback_secs
+        timing["switched_off"] = switched
+    if switched:
+        log(f"  network switched off (runaway signal speed; plain coarse solver from then on) for "
+            f"{len(switched)} instance(s): {switched[:8]}")
     return np.concatenate(frames_all), secs, fell
 
 
@@ -349,13 +360,14 @@ def matched_speedup(ns, errs, times, e_target, t_learned):
     return n_star, t_star, t_star / t_learned
 
 
-def evaluate(model, test_idx, truth64, ladder=(64, 128, 256), log=print, n=COARSE, n_fine=FINE):
+def evaluate(model, test_idx, truth64, ladder=(64, 128, 256), log=print, n=COARSE, n_fine=FINE, batch=64):
     """The matched-accuracy comparison on held-out instances: the plain solver at every ladder resolution and
     the learned solver at n, all scored at n against truth64 (the n_fine truth block-averaged to n, frames
-    (N, NOUT+1, 4, n, n)); the learned row is stored under "learned64" for n = 64, else "learned<n>"."""
-    res = {"test_instances": len(test_idx), "n": n, "n_fine": n_fine, "plain": {}}
+    (N, NOUT+1, 4, n, n)), every solver run on batches of `batch` instances; the learned row is stored under
+    "learned64" for n = 64, else "learned<n>"."""
+    res = {"test_instances": len(test_idx), "n": n, "n_fine": n_fine, "batch": batch, "plain": {}}
     for k in ladder:
-        fr, secs = plain_rollout(test_idx, k, m=min(k, n), n_fine=n_fine)
+        fr, secs = plain_rollout(test_idx, k, m=min(k, n), n_fine=n_fine, batch=batch)
         if k < n:  # scored piecewise constant on the n grid
             f = n // k
             fr = np.repeat(np.repeat(fr, f, axis=-2), f, axis=-1)
@@ -365,12 +377,13 @@ def evaluate(model, test_idx, truth64, ladder=(64, 128, 256), log=print, n=COARS
         res["plain"][k] = {"l1_rho": e, "seconds": secs}
         log(f"  plain {k:4d}^2: L1(rho) {e:.5f}  wall {secs:.3f} s")
     timing = {}
-    fr, secs, fell = learned_rollout(model, test_idx, n=n, log=log, timing=timing, n_fine=n_fine)
+    fr, secs, fell = learned_rollout(model, test_idx, n=n, log=log, timing=timing, n_fine=n_fine, batch=batch)
     e = float(l1_density(fr, truth64)[:, 1:].mean())
     own = np.array([i not in fell for i in test_idx])
     e_own = float(l1_density(fr[own], truth64[own])[:, 1:].mean())
     e_plain_own = float(l1_density(plain_n[own], truth64[own])[:, 1:].mean())
     res[f"learned{n}"] = {"l1_rho": e, "seconds": secs, "fallbacks": len(fell), "fallback_seconds": timing["fallback"],
+                          "network_switched_off": len(timing["switched_off"]),
                           "l1_rho_without_fallback_instances": e_own,
                           f"plain{n}_l1_rho_same_instances": e_plain_own}
     log(f"  learned {n}^2: L1(rho) {e:.5f}  wall {secs:.3f} s  fallbacks {len(fell)}  "

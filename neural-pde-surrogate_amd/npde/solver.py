@@ -312,6 +312,18 @@ class Batch:
         check(self._h, self._lib.npde_set_flux_network(self._h, C.c_void_p(frags.data_ptr()),
                                                        C.c_void_p(biases.data_ptr()), scale))
 
+    def set_flux_network_limit(self, ratio: float):
+        """Switch the network off for an instance (zero factors: the plain coarse solver) at the first frame
+        landing of a run where its CFL signal-speed sum exceeds `ratio` times its first frame's; 0 never
+        (default 2.5)."""
+        check(self._h, self._lib.npde_set_flux_network_limit(self._h, float(ratio)))
+
+    def flux_network_switched_off(self) -> np.ndarray:
+        """Per instance, True if the last run switched its network off."""
+        out = np.zeros(self.n, dtype=np.int32)
+        check(self._h, self._lib.npde_flux_network_switched_off(self._h, out.ctypes.data_as(C.POINTER(C.c_int))))
+        return out.astype(bool)
+
     def flux_network_eval(self):
         """Evaluate the network now on every instance's current state (its factors for the next steps)."""
         check(self._h, self._lib.npde_flux_network_eval(self._h))

@@ -411,6 +411,7 @@ def test_network_in_the_device_loop_equals_a_host_loop(npde_mod, cuda):
             a.set_state(k, U0[k])
             h.set_state(k, U0[k])
         a.set_flux_network(m)
+        a.set_flux_network_limit(0)  # the host loop below has no switch-off
         h.set_flux_network(m)
         a.run_trajectory(T, NOUT, sizes=(n,))
         frames = {k: a.trajectory(k, n) for k in range(B)}
@@ -446,3 +447,60 @@ def test_accuracy_cost_table(npde_mod, cuda):
         beaten = any(o is not r and o["cost"] <= r["cost"] and o["rmse_last"] <= r["rmse_last"]
                      and (o["cost"] < r["cost"] or o["rmse_last"] < r["rmse_last"]) for o in rows.values())
         assert r["pareto"] == (not beaten)
+
+
+@pytest.mark.gpu
+def test_network_switches_off_where_the_signal_speed_runs_away(npde_mod, cuda):
+    """npde_set_flux_network_limit: at a frame landing where an instance's CFL signal-speed sum exceeds the
+    limit times its first frame's, its factors become zero -- from then on it is the plain coarse solver.
+    With a limit every instance exceeds at once, the in-loop run equals a host loop that evaluates the network
+    at t0, advances to the first frame and then runs without it to every later frame time, bit for bit; with
+    limit 0 nothing switches.
+    The held-out instance 100511, whose corrected flow drains a cell toward vacuum (1,723 steps against 695
+    plain without the switch), is caught by the default limit and finishes in far fewer steps (the drained
+    cell, already there when the limit trips, keeps its steps above the plain solver's)."""
+    from npde.learned import pipeline as P
+    from npde.learned.model import load_trained
+    m = _random_net(7, cuda, head_scale=0.3)
+    m.out_scale = 0.1
+    n, B, T, NOUT = 64, 3, 0.04, 4
+    U0 = riemann_batch(n, range(30, 30 + B))
+    grids = [npde_mod.make_grid(n, n, bc_x=1, bc_y=1)] * B
+    with npde_mod.Batch(grids, "hllc", "vanleer", 0.5) as a, npde_mod.Batch(grids, "hllc", "vanleer", 0.5) as h:
+        for k in range(B):
+            a.set_state(k, U0[k])
+            h.set_state(k, U0[k])
+        a.set_flux_network(m)
+        a.set_flux_network_limit(1e-9)
+        a.run_trajectory(T, NOUT, sizes=(n,))
+        assert a.flux_network_switched_off().all()
+        h.set_flux_network(m)
+        h.flux_network_eval()
+        h.advance_to(T * 1 / NOUT)
+        h.set_flux_network(None)
+        for j in range(2, NOUT + 1):  # landing on every frame time, as the recorded run does
+            h.advance_to(T * j / NOUT)
+        for k in range(B):
+            assert a.time(k) == h.time(k)
+            assert np.array_equal(a.get_state(k).view(np.uint64), h.get_state(k).view(np.uint64)), k
+        a.set_flux_network_limit(0)
+        for k in range(B):
+            a.set_state(k, U0[k])
+        a.run_trajectory(T, NOUT, sizes=(n,))
+        assert not a.flux_network_switched_off().any()
+    U = P.block_average_f64(P.riemann(100511, 512)[0], 64)[None]
+    g = [npde_mod.make_grid(64, 64, bc_x=1, bc_y=1)]
+    with npde_mod.Batch(g, "hllc", "vanleer", 0.5) as b:
+        b.set_state(0, U[0])
+        b.run_trajectory(0.5, 50, sizes=(64,))
+        plain = int(b.times()[1][0])
+        b.set_flux_network(load_trained(device=cuda))
+        steps = {}
+        for limit in (0.0, 2.5):
+            b.set_flux_network_limit(limit)
+            b.set_state(0, U[0])
+            b.run_trajectory(0.5, 50, sizes=(64,))
+            steps[limit] = (int(b.times()[1][0]), bool(b.flux_network_switched_off()[0]))
+    print(f"instance 100511: plain {plain} steps, learned {steps}", flush=True)
+    assert steps[0.0][0] > 1.5 * plain and not steps[0.0][1]
+    assert steps[2.5][1] and steps[2.5][0] < 0.7 * steps[0.0][0]
