@@ -26,14 +26,15 @@ hipError_t stage_occupancy_fast(int scheme, int lim, bool corr, int waves_per_bl
                                 unsigned seam_bytes, unsigned lds_reserve, int* workgroups);
 
 
-// Reduce an instance's per-task wave-speed partials (one wavefront), complete the step in flight and
-// decide the next one.  Runs in its own launch after the kernel that wrote the partials, so the
-// kernel boundary orders the partials before their reduction: no device-scope fence (which on this
-// chip writes back and invalidates a whole L2) is needed inside the stage kernels.
-__device__ void finish_instance(const InstDesc& D, Ctl* c, const Partial* __restrict__ part, int lane,
-                                int* ndone) {
-    double sx = 0.0, sy = 0.0;
-    int bad = kNoBad;
+// Reduce an instance's per-task wave-speed partials (one wavefront; every lane gets the result).  Runs in
+// a launch after the kernel that wrote the partials, so the kernel boundary orders the partials before
+// their reduction: no device-scope fence (which on this chip writes back and invalidates a whole L2) is
+// needed inside the stage kernels.
+__device__ void reduce_partials(const InstDesc& D, const Partial* __restrict__ part, int lane, double& sx,
+                                double& sy, int& bad) {
+    sx = 0.0;
+    sy = 0.0;
+    bad = kNoBad;
     for (int t = lane; t < D.ntasks; t += kWave) {
         const Partial p = part[D.task0 + t];
         sx = tmax(sx, p.sx);
@@ -43,17 +44,15 @@ __device__ void finish_instance(const InstDesc& D, Ctl* c, const Partial* __rest
     sx = wave_max(sx);
     sy = wave_max(sy);
     bad = wave_min(bad);
-    if (lane == 0) {
-        c->sx = sx;
-        c->sy = sy;
-        c->bad = bad;
-        if (c->active) {
-            c->t = c->t + c->dt;
-            c->steps += 1;
-            c->active = 0;
-        }
-        if (c->mode == kModeAuto) decide_next(D, c, ndone);
-    }
+}
+
+// Complete the step in flight and decide the next move (one wavefront).
+__device__ void finish_instance(const InstDesc& D, Ctl* c, const Partial* __restrict__ part, int lane,
+                                int* ndone) {
+    double sx, sy;
+    int bad;
+    reduce_partials(D, part, lane, sx, sy, bad);
+    if (lane == 0 && finish_step(D, c, sx, sy, bad)) atomicAdd(ndone + kCountDone, 1);
 }
 
 enum FinishMode : int { kAfterStep = 0, kAfterWavespeed = 1 };
@@ -123,50 +122,76 @@ __global__ __launch_bounds__(kMaxRowWaves * kWave) void k_wavespeed(const InstDe
     publish_partial(part, tid, lane, sx, sy, bad);
 }
 
-// Block averages of the conserved variables for instances with a pending frame
-// (trajectory.hpp:39-48: sum over the f x f block, rows then columns, from 0.0; times 1/(f*f); to float).
-// The last of an instance's frame workgroups to finish completes the landing (complete_landing; hold:
-// a flux-network run).  Every workgroup of the instance read `pending` before it counted itself in
-// Ctl::fdone, so the landing's update of the run state cannot overtake a reader.
+// One launch per iteration of a recording run, in place of the previous iteration's k_finish: with
+// `complete` (every iteration but a launch sequence's first, whose predecessor ended with a k_finish),
+// every frame workgroup of an instance first completes the step the previous iteration's stage kernels
+// took (Ctl::active) -- each computes the same reduction and run-state decision from the same inputs into
+// its own copy -- then, if that (or an earlier) decision landed on a frame, writes its share of the block
+// averages (trajectory.hpp:39-48: sum over the
+// f x f block, rows then columns, from 0.0; times 1/(f*f); to float).  The last of the instance's frame
+// workgroups to finish stores the new run state and completes the landing (complete_landing; hold: a
+// flux-network run).  Every workgroup read the run state before it counted itself in Ctl::fdone, so that
+// store cannot overtake a reader.
 __global__ __launch_bounds__(256) void k_frame(const InstDesc* __restrict__ desc, Ctl* __restrict__ ctl,
-                                               const FTask* __restrict__ ftasks, int nftasks, int hold,
-                                               int* __restrict__ ndone) {
+                                               const Partial* __restrict__ part, const FTask* __restrict__ ftasks,
+                                               int nftasks, int complete, int hold, int* __restrict__ ndone) {
     const int bid = blockIdx.x;
     if (bid >= nftasks) return;
     const FTask ft = ftasks[bid];
     Ctl* c = ctl + ft.inst;
-    const int k = c->pending;
-    if (k < 0) return;
     const InstDesc& D = desc[ft.inst];
-    const int lv = ft.level, m = D.lvl_m[lv], f = D.lvl_f[lv];
-    const int nyc = D.ny / f;
-    const int cell = ft.c0 + threadIdx.x;
-    if (cell < m * nyc) {
-        const int jc = cell / m, ic = cell - jc * m;
-        const long long P = D.plane;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        for (int j = jc * f; j < (jc + 1) * f; ++j) {
-            const long long rowoff = (long long)j * D.nx;
-            for (int ii = ic * f; ii < (ic + 1) * f; ++ii) {
-                const long long o = rowoff + ii;
-                s0 += D.U[o];
-                s1 += D.U[o + P];
-                s2 += D.U[o + 2 * P];
-                s3 += D.U[o + 3 * P];
-            }
+    __shared__ Ctl s_c;       // the instance's run state after this launch
+    __shared__ int s_newly;   // 1 when the step completion finished the instance
+    if (threadIdx.x < kWave) {
+        const int lane = threadIdx.x;
+        Ctl cl = *c;
+        int newly = 0;
+        if (complete && cl.active) {
+            double sx, sy;
+            int bad;
+            reduce_partials(D, part, lane, sx, sy, bad);
+            newly = finish_step(D, &cl, sx, sy, bad);
         }
-        const double inv = 1.0 / (double(f) * f);
-        float* out = D.traj[lv] + (size_t)k * 4 * nyc * m;
-        const size_t pl = (size_t)nyc * m;
-        out[cell] = float(s0 * inv);
-        out[pl + cell] = float(s1 * inv);
-        out[2 * pl + cell] = float(s2 * inv);
-        out[3 * pl + cell] = float(s3 * inv);
+        if (lane == 0) {
+            s_c = cl;
+            s_newly = newly;
+        }
     }
     __syncthreads();
+    const int k = s_c.pending;
+    if (k >= 0) {
+        const int lv = ft.level, m = D.lvl_m[lv], f = D.lvl_f[lv];
+        const int nyc = D.ny / f;
+        const int cell = ft.c0 + threadIdx.x;
+        if (cell < m * nyc) {
+            const int jc = cell / m, ic = cell - jc * m;
+            const long long P = D.plane;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            for (int j = jc * f; j < (jc + 1) * f; ++j) {
+                const long long rowoff = (long long)j * D.nx;
+                for (int ii = ic * f; ii < (ic + 1) * f; ++ii) {
+                    const long long o = rowoff + ii;
+                    s0 += D.U[o];
+                    s1 += D.U[o + P];
+                    s2 += D.U[o + 2 * P];
+                    s3 += D.U[o + 3 * P];
+                }
+            }
+            const double inv = 1.0 / (double(f) * f);
+            float* out = D.traj[lv] + (size_t)k * 4 * nyc * m;
+            const size_t pl = (size_t)nyc * m;
+            out[cell] = float(s0 * inv);
+            out[pl + cell] = float(s1 * inv);
+            out[2 * pl + cell] = float(s2 * inv);
+            out[3 * pl + cell] = float(s3 * inv);
+        }
+    }
     if (threadIdx.x == 0 && atomicAdd(&c->fdone, 1) == D.nftasks - 1) {
-        c->fdone = 0;
-        complete_landing(D, c, hold, ndone);
+        Ctl cl = s_c;
+        cl.fdone = 0;
+        *c = cl;
+        if (s_newly) atomicAdd(ndone + kCountDone, 1);
+        if (cl.pending >= 0) complete_landing(D, c, hold, ndone);
     }
 }
 
@@ -329,10 +354,11 @@ hipError_t launch_wavespeed(int waves_per_block, int mode, int row_waves, const 
     return hipGetLastError();
 }
 
-hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int nftasks, int hold, int* ndone,
-                        hipStream_t s) {
+hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const Partial* part, const FTask* ftasks, int nftasks,
+                        int complete, int hold, int* ndone, hipStream_t s) {
     if (nftasks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_frame, dim3(nftasks), dim3(256), 0, s, desc, ctl, ftasks, nftasks, hold, ndone);
+    hipLaunchKernelGGL(k_frame, dim3(nftasks), dim3(256), 0, s, desc, ctl, part, ftasks, nftasks, complete, hold,
+                       ndone);
     return hipGetLastError();
 }
 

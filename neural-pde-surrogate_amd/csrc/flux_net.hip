@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kNetThreads) void k_flux_net(const InstDesc* __rest
         Ctl* c = ctl + k;
         if (!c->hold) continue;
         c->hold = 0;
-        decide_next(desc[k], c, counters);
+        if (decide_next(desc[k], c)) atomicAdd(counters + kCountDone, 1);
     }
     if (threadIdx.x == 0) {
         counters[kCountHeld] = 0;

@@ -30,8 +30,8 @@ hipError_t stage_occupancy(int scheme, int lim, bool corr, bool fast, int waves_
                            int seam_rows, unsigned lds_reserve, int* workgroups);
 hipError_t launch_wavespeed(int waves_per_block, int mode, int row_waves, const InstDesc* desc, Ctl* ctl,
                             const Task* tasks, int ntasks, Partial* part, int* ndone, hipStream_t s);
-hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const FTask* ftasks, int nftasks, int hold, int* ndone,
-                        hipStream_t s);
+hipError_t launch_frame(const InstDesc* desc, Ctl* ctl, const Partial* part, const FTask* ftasks, int nftasks,
+                        int complete, int hold, int* ndone, hipStream_t s);
 hipError_t launch_block_average(const InstDesc* desc, int inst, int m, int nyc, float* out, hipStream_t s);
 hipError_t launch_finish(const InstDesc* desc, Ctl* ctl, const Partial* part, int n, int mode, int* ndone,
                          hipStream_t s);
@@ -534,13 +534,18 @@ npde_status launch_wavespeed_and_finish(npde_solver* s) {
     return launch_finish_kernel(s, kAfterWavespeed);
 }
 
+// `iters` iterations of the device loop.  Without frames: stage 1, stage 2, k_finish.  With frames (nft
+// frame tasks): the frame kernel, which completes the previous iteration's step (in place of its k_finish;
+// not in the first iteration, whose predecessor is a k_finish: a step it decided is not taken yet) and
+// writes the frames of the instances that landed, [the network,] stage 1, stage 2 -- and one k_finish after
+// the last iteration, so that every launch sequence ends with its steps completed.
 npde_status launch_iterations(npde_solver* s, int iters, int nft) {
     for (int it = 0; it < iters; ++it) {
         if (nft > 0) {
-            // the frame kernel writes the frames of the instances that landed and completes their landings
             const int hold = s->net_frags ? 1 : 0;
             npde_status st = timed_launch(s, NPDE_PROF_FRAME, [&] {
-                return launch_frame(s->ddesc, s->dctl, s->dftasks, nft, hold, s->dndone, s->stream);
+                return launch_frame(s->ddesc, s->dctl, s->dpart, s->dftasks, nft, it > 0 ? 1 : 0, hold, s->dndone,
+                                    s->stream);
             });
             // flux-network run: an instance that reached its frame holds there; once every running instance
             // holds, the network turns their states into the next interval's factors (one launch for the
@@ -550,11 +555,14 @@ npde_status launch_iterations(npde_solver* s, int iters, int nft) {
                     return launch_flux_net(s->ddesc, s->dctl, s->dnettasks, s->nnettasks, s->n, s->dndone,
                                            s->net_frags, s->net_bias, s->net_scale, s->dnetpeak, s->stream);
                 });
+            for (int stage = 1; stage <= 2 && st == NPDE_OK; ++stage) st = launch_stage_kernel(s, stage, it == 0);
+            if (st != NPDE_OK) return st;
+        } else {
+            npde_status st = launch_step_pair(s, it == 0);
             if (st != NPDE_OK) return st;
         }
-        npde_status st = launch_step_pair(s, it == 0);
-        if (st != NPDE_OK) return st;
     }
+    if (nft > 0 && iters > 0) return launch_finish_kernel(s, kAfterStep);
     return NPDE_OK;
 }
 

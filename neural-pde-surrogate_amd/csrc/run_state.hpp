@@ -1,5 +1,5 @@
-// Run-state machine of a device-resident run, shared by the kernels that move an instance on: k_finish
-// (euler_kernels.hip), k_frame (euler_kernels.hip) and, in flux-network runs, k_flux_net (flux_net.hip).
+// Run-state machine of a device-resident run, shared by the kernels that move an instance on: k_finish and
+// k_frame (euler_kernels.hip) and, in flux-network runs, k_flux_net (flux_net.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -18,38 +18,51 @@ __device__ __forceinline__ double target_time(const Ctl* c, int k) {
     return c->t_start + (c->t_end - c->t_start) * double(k) / double(c->nout);
 }
 
-__device__ inline void decide_next(const InstDesc& D, Ctl* c, int* ndone) {
-    if (c->done) return;
+// The instance's next move; returns 1 when this call finished it (the caller counts it in
+// counters[kCountDone]).
+__device__ inline int decide_next(const InstDesc& D, Ctl* c) {
+    if (c->done) return 0;
     for (;;) {
         if (c->k > c->nout) {
             c->done = 1;
-            atomicAdd(ndone, 1);
-            return;
+            return 1;
         }
         const double T = target_time(c, c->k);
         if (c->t < T) {
             if (c->steps >= c->step_limit) {  // step budget of npde_advance_steps spent
                 c->done = 1;
-                atomicAdd(ndone, 1);
-                return;
+                return 1;
             }
             if (c->bad != kNoBad) {
                 c->error = 1;
                 c->done = 1;
-                atomicAdd(ndone, 1);
-                return;
+                return 1;
             }
             const double dt = D.cfl / (c->sx / D.dx + c->sy / D.dy);
             c->dt = tmin(dt, T - c->t);
             c->active = 1;
-            return;
+            return 0;
         }
         if (c->record) {
             c->pending = c->k;
-            return;
+            return 0;
         }
         c->k += 1;
     }
+}
+
+// Completion of the step in flight (c->active) from the reduced wave-speed partials of the new state,
+// then (device-decided runs) the next move; returns 1 when it finished the instance.
+__device__ inline int finish_step(const InstDesc& D, Ctl* c, double sx, double sy, int bad) {
+    c->sx = sx;
+    c->sy = sy;
+    c->bad = bad;
+    if (c->active) {
+        c->t = c->t + c->dt;
+        c->steps += 1;
+        c->active = 0;
+    }
+    return c->mode == kModeAuto ? decide_next(D, c) : 0;
 }
 
 // A frame landing is complete (its frame written): record its time and move to the next target.  hold: a
@@ -75,7 +88,7 @@ __device__ inline void complete_landing(const InstDesc& D, Ctl* c, int hold, int
         atomicAdd(counters + kCountHeld, 1);
         return;
     }
-    decide_next(D, c, counters);
+    if (decide_next(D, c)) atomicAdd(counters + kCountDone, 1);
 }
 
 }  // namespace npde

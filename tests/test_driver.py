@@ -79,7 +79,9 @@ def test_profile_table_splits_the_run_by_kernel(npde_mod, tmp_path):
     rows = {m.group(1).strip(): (float(m.group(2)), int(m.group(3)) if m.group(3) else None)
             for m in re.finditer(r"^(\S.*?)\s+([0-9.]+)\s+[0-9.]+%\s+[0-9.]+(?:\s+(\d+))?$", p.stdout, re.M)}
     assert rows["stage 1 (slopes, fluxes, U1)"][1] == rows["stage 2 (+ U^n+1, wave speeds)"][1] >= steps
-    assert rows["dt reduction + run state"][1] > rows["stage 2 (+ U^n+1, wave speeds)"][1]
+    # every step is completed by a k_finish or, in launches with frames, by the next frame kernel
+    assert (rows["dt reduction + run state"][1] + rows["frames (block averages)"][1]
+            > rows["stage 2 (+ U^n+1, wave speeds)"][1])
     assert rows["frames (block averages)"][1] >= 5
     assert rows["stage 1 (slopes, fluxes, U1)"][0] > 0 and rows["stage 2 (+ U^n+1, wave speeds)"][0] > 0
     for k in ("launch gaps (device loop)", "output (copy + HDF5)", "host (setup, sync)", "total"):
@@ -109,7 +111,8 @@ def test_profiled_batch_matches_the_unprofiled_run(npde_mod):
             pr = b.profile()
             if prof:
                 assert pr["stage1"][1] == pr["stage2"][1] >= res[-1][2].max()
-                assert pr["finish"][1] >= pr["stage2"][1] and pr["frame"][1] >= 6 and pr["wavespeed"][1] >= 1
+                assert pr["finish"][1] + pr["frame"][1] >= pr["stage2"][1]
+                assert pr["frame"][1] >= 6 and pr["wavespeed"][1] >= 1
                 assert pr["stage1"][0] > 0 and pr["stage2"][0] > 0 and pr["network"] == (0.0, 0)
                 b.set_profiling(True)
                 assert all(v == (0.0, 0) for v in b.profile().values())
