@@ -76,6 +76,10 @@ typedef struct {
                              (60-column strips with a 2-column halo) */
     int waves_per_simd;   /* occupancy cap of the stage kernels: at most this many resident wavefronts per SIMD,
                              enforced by reserving LDS per workgroup (0: as many as registers and LDS allow) */
+    int net_period;       /* flux-network runs: the network kernel is launched in every net_period-th iteration
+                             of the device loop and in the last iteration of every launch (1: every iteration;
+                             1..64, default 4).  Instances that land on a frame hold until it runs, so the
+                             period changes only time, never a result */
 } npde_tuning;
 
 typedef struct npde_solver npde_solver;

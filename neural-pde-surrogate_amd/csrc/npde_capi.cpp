@@ -52,6 +52,9 @@ using namespace npde;
 // steps_per_graph itself), one graph per size: level l runs min(2^l, steps_per_graph) iterations.
 constexpr int kGraphLevels = 32;
 
+// Flux-network runs: the network kernel's launch period in device-loop iterations (npde_tuning::net_period).
+constexpr int kNetPeriod = 4;
+
 struct npde_solver {
     int n = 0;
     int flux = kHllc, lim = kLimVanLeer;
@@ -87,7 +90,7 @@ struct npde_solver {
     std::vector<int> prof_kind;           // kind of each recorded launch not yet read
     double prof_ms[NPDE_PROF_KINDS] = {};
     long long prof_n[NPDE_PROF_KINDS] = {};
-    npde_tuning tune{0, 4, 32, 1, 1, 1, 0};
+    npde_tuning tune{0, 4, 32, 1, 1, 1, 0, kNetPeriod};
     int lds_per_cu = 65536;    // LDS bytes per CU of the handle's device (queried at creation)
     int layout = kStrips;  // StageMode of the current task list
     int row_waves = 0;   // row groups: wavefronts per band
@@ -549,8 +552,12 @@ npde_status launch_iterations(npde_solver* s, int iters, int nft) {
             });
             // flux-network run: an instance that reached its frame holds there; once every running instance
             // holds, the network turns their states into the next interval's factors (one launch for the
-            // batch), and its last workgroup lets them all move on
-            if (st == NPDE_OK && hold)
+            // batch), and its last workgroup lets them all move on.  The network kernel is launched in every
+            // net_period-th iteration, counted back from the launch's last one (so every launch, of any
+            // length, can release a held batch): a batch that holds waits at most net_period - 1 iterations
+            // in which nothing steps, and most iterations are spared a launch that would find the gate shut.
+            const int period = std::max(1, s->tune.net_period);
+            if (st == NPDE_OK && hold && (iters - 1 - it) % period == 0)
                 st = timed_launch(s, NPDE_PROF_NETWORK, [&] {
                     return launch_flux_net(s->ddesc, s->dctl, s->dnettasks, s->nnettasks, s->n, s->dndone,
                                            s->net_frags, s->net_bias, s->net_scale, s->dnetpeak, s->stream);
@@ -1293,7 +1300,8 @@ npde_status npde_get_tuning(const npde_solver* s, npde_tuning* t) {
 npde_status npde_set_tuning(npde_solver* s, const npde_tuning* t) {
     if (!s || !t) return NPDE_ERR_INVALID;
     if (t->rows_per_task < 0 || !(t->waves_per_block == 1 || t->waves_per_block == 2 || t->waves_per_block == 4) ||
-        t->steps_per_graph < 1 || t->waves_per_simd < 0 || t->waves_per_simd > 8)
+        t->steps_per_graph < 1 || t->waves_per_simd < 0 || t->waves_per_simd > 8 || t->net_period < 1 ||
+        t->net_period > 64)
         return fail(s, NPDE_ERR_INVALID, "bad tuning");
     // an occupancy cap must be reachable with a launch's LDS (checked for the layout it gives; row groups take
     // their workgroup size from the width)

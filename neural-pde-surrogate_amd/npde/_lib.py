@@ -27,7 +27,8 @@ class BadCell(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("rows_per_task", C.c_int), ("waves_per_block", C.c_int), ("steps_per_graph", C.c_int),
-                ("use_graph", C.c_int), ("row_groups", C.c_int), ("seams", C.c_int), ("waves_per_simd", C.c_int)]
+                ("use_graph", C.c_int), ("row_groups", C.c_int), ("seams", C.c_int), ("waves_per_simd", C.c_int),
+                ("net_period", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -425,6 +425,22 @@ def test_network_in_the_device_loop_equals_a_host_loop(npde_mod, cuda):
         for k in range(B):
             assert a.time(k) == h.time(k)
             assert np.array_equal(a.get_state(k), h.get_state(k))
+        final = {k: a.get_state(k) for k in range(B)}
+        # the network kernel's launch period and the graph length change only when a held batch is released,
+        # never a result: every period (including ones longer than some launches) gives the same bits
+        assert a.tuning["net_period"] == 4  # the default
+        for period, graph in ((1, 32), (3, 32), (7, 5), (64, 8), (2, 1)):
+            a.set_tuning(net_period=period, steps_per_graph=graph)
+            for k in range(B):
+                a.set_state(k, U0[k])
+            a.run_trajectory(T, NOUT, sizes=(n,))
+            for k in range(B):
+                assert np.array_equal(a.trajectory(k, n), frames[k]), (period, graph, k)
+                assert np.array_equal(a.get_state(k), final[k]), (period, graph, k)
+            assert np.array_equal(a.times()[1], steps), (period, graph)
+        for bad in (0, 65):
+            with pytest.raises(npde_mod.NpdeError):
+                a.set_tuning(net_period=bad)
 
 
 @pytest.mark.gpu
