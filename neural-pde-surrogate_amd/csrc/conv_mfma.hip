@@ -1,25 +1,37 @@
-// 3x3 periodic convolution on BF16 MFMA with FP32 accumulation (gfx950), for the learned flux
-// correction network.  No counterpart in the reference (its networks are JAX); the closest analog is
-// the periodic 3x3 convolution of surrogate/models.py:16-29.
+// 3x3 periodic convolution on BF16 MFMA with FP32 accumulation (gfx950): the layer kernels of the learned
+// flux correction's training path and of the full-state surrogates (ResNet, U-Net).  No counterpart in the
+// reference (its networks are JAX); the closest analog is the periodic 3x3 convolution of
+// surrogate/models.py:16-29.
 //
-// Layout: activations NHWC float32 in memory, channel count a multiple of 16; every value is rounded
-// to bfloat16 (round to nearest even) when it enters an MFMA, so the arithmetic is BF16 x BF16 products
-// accumulated in FP32.  Weights are pre-packed as BF16 in the MFMA operand order
-// [tap][ci/16][co/16][lane][4] (tap = (dy+1)*3 + (dx+1), cross-correlation, circular padding).
+// Activations are NHWC float32 in memory (channel count a multiple of 8); every value enters an MFMA
+// rounded to bfloat16 (round to nearest even), so the arithmetic is BF16 x BF16 products accumulated in
+// FP32.
 //
-// Implicit GEMM with v_mfma_f32_16x16x16_bf16 (A 16x16 M x K, B 16x16 K x N, D 16x16; lane l holds
-// A[l%16][4(l/16)..+3], B[4(l/16)..+3][l%16], D[4(l/16)..+3][l%16]):
-//   k_conv_fwd   M = 16 pixels of one image row, N = output channels, K = 9 taps x input channels;
-//                epilogue: + bias, optional tanh-GELU, optional pre-activation output; or, for the
-//                data gradient, multiplication by GELU'(pre-activation of the previous layer)
-//   k_conv_wgrad M = input channels of one tap, N = output channels, K = pixels of a band of rows;
-//                one partial per (band, tap) -> k_wgrad_reduce sums the bands in a fixed order
+// k_conv_fwd (forward and data gradient): an implicit GEMM on v_mfma_f32_16x16x32_bf16.  A workgroup of
+// 4 wavefronts owns a tile of up to 128 output pixels of one image (8 x 16, 16 x 8 or 8 x 8 rows x
+// columns).  For every 32 input channels it stages the tile's input region -- the tile and a 1-pixel
+// periodic halo -- in LDS as bfloat16, once (64-byte pixel records, swizzled against bank conflicts), and
+// the wavefronts then take all nine taps from there: M = 16 output channels (the weights, pre-packed in
+// MFMA operand order, are the A operand), N = 16 pixels of the tile, K = the 32 staged channels of one tap
+// (their B operand, one 16-byte LDS read per lane).  Epilogue: + bias, optional tanh-GELU, optional
+// pre-activation output; or, for the data gradient, multiplication by GELU'(pre-activation of the
+// previous layer).  Weights: [tap][cin/32][cout/16][lane][8] (tap = (dy+1)*3 + (dx+1), cross-correlation,
+// circular padding; channels beyond cin or cout zero).
+//
+// k_conv_wgrad: M = input channels of one tap, N = output channels, K = pixels of a band of rows, on
+// v_mfma_f32_16x16x16_bf16; one partial per (band, tap) -> k_wgrad_reduce sums the bands in a fixed order.
 #include <hip/hip_runtime.h>
+
+#include <cstdint>
 
 namespace npde_nn {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ short bf16_rne(float f) {
     unsigned u = __float_as_uint(f);
@@ -35,6 +47,11 @@ __device__ __forceinline__ v4s to_bf16x4(float a, float b, float c, float d) {
     r.z = bf16_rne(c);
     r.w = bf16_rne(d);
     return r;
+}
+
+// two floats -> two bfloat16 (round to nearest even) in one dword, low half first (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ unsigned pack_bf16(float lo, float hi) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(f32x2{lo, hi}, bf16x2));
 }
 
 constexpr float kGeluC0 = 0.7978845608028654f;  // sqrt(2/pi)
@@ -54,111 +71,179 @@ __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >=
 
 enum Epilogue : int { kLinear = 0, kGelu = 1, kGeluGrad = 2 };
 
-// One wavefront: 16 consecutive pixels of one image row x all output channels (NCOT tiles of 16).
-template <int NCOT>
-__global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ x, int B, int H, int W, int Cin,
-                                                  const v4s* __restrict__ wp, const float* __restrict__ bias,
-                                                  int cout_store, float* __restrict__ y, float* __restrict__ pre,
-                                                  int epi, const float* __restrict__ gpre) {
+constexpr int kConvThreads = 256;
+constexpr int kConvWaves = kConvThreads / 64;
+constexpr int kTilePixels = 128;          // output pixels per workgroup at most
+constexpr int kRegionMax = 18 * 10;       // staged pixels: (rows + 2) x (columns + 2) of an 8 x 16 or 16 x 8 tile
+constexpr int kMaxBlocksPerWave = kTilePixels / 16 / kConvWaves;
+
+// LDS pixel records: 32 bfloat16 channels = four 16-byte pieces; piece c of pixel p sits at c ^ ((p >> 1) & 3)
+// (in units of 8 channels), so that the 16 lanes of a group, reading piece g of 16 consecutive pixels, spread
+// over the banks.  Offsets in bfloat16 elements.
+__device__ __forceinline__ int rec_at(int p, int c) { return (p << 5) + ((c ^ ((p >> 1) & 3)) << 3); }
+
+// One workgroup per tile of TH x TW output pixels (TH * TW a multiple of 16, at most 128; H % TH == 0,
+// W % TW == 0); NT = cout / 16 output-channel tiles.
+template <int NT>
+__global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const float* __restrict__ x, int B, int H, int W,
+                                                           int cin, int TH, int TW, const u32x4* __restrict__ wp,
+                                                           const float* __restrict__ bias, int cout_store,
+                                                           float* __restrict__ y, float* __restrict__ pre, int epi,
+                                                           const float* __restrict__ gpre) {
+    __shared__ __align__(16) unsigned short s_in[kRegionMax * 32];
+    const int tiles_x = W / TW, tiles_y = H / TH;
+    const int tile = blockIdx.x;
+    const int b = tile / (tiles_x * tiles_y);
+    if (b >= B) return;
+    const int y0 = (tile / tiles_x) % tiles_y * TH, x0 = tile % tiles_x * TW;
+    const int RW = TW + 2, nreg = (TH + 2) * RW;
     const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int xblocks = W >> 4;
-    if (wave >= B * H * xblocks) return;
-    const int xb = wave % xblocks;
-    const int row = (wave / xblocks) % H;
-    const int b = wave / (xblocks * H);
-    const int m = lane & 15, kq = lane >> 4;
-    const int x0 = xb << 4;
-    const int nc16 = Cin >> 4;
-    v4f acc[NCOT];
-    for (int c = 0; c < NCOT; ++c) acc[c] = v4f{0.f, 0.f, 0.f, 0.f};
-    for (int t = 0; t < 9; ++t) {
-        const int yy = wrap(row + t / 3 - 1, H);
-        const int xx = wrap(x0 + m + t % 3 - 1, W);
-        const float* src = x + ((size_t(b) * H + yy) * W + xx) * Cin + kq * 4;
-        for (int c16 = 0; c16 < nc16; ++c16) {
-            const float4 a4 = *reinterpret_cast<const float4*>(src + c16 * 16);
-            const v4s a = to_bf16x4(a4.x, a4.y, a4.z, a4.w);
-            const v4s* wt = wp + (size_t(t * nc16 + c16) * NCOT) * 64 + lane;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int m = lane & 15, g = lane >> 4;
+    const int nmb = TH * TW / 16;
+    const int nchunk = (cin + 31) >> 5;
+
+    v4f acc[kMaxBlocksPerWave][NT];
 #pragma unroll
-            for (int c = 0; c < NCOT; ++c) acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wt[c * 64], acc[c], 0, 0, 0);
-        }
+    for (int i = 0; i < kMaxBlocksPerWave; ++i)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[i][t] = v4f{0.f, 0.f, 0.f, 0.f};
+    // region pixel of this lane's tile pixel in each of its M blocks (tap 0); the tap adds dy * RW + dx
+    int rp0[kMaxBlocksPerWave];
+#pragma unroll
+    for (int i = 0; i < kMaxBlocksPerWave; ++i) {
+        const int pix = (wave + kConvWaves * i) * 16 + m;
+        const int py = pix / TW;
+        rp0[i] = py * RW + (pix - py * TW);
     }
-    const int cout = NCOT * 16;
+
+    for (int c = 0; c < nchunk; ++c) {
+        // stage channels 32c .. 32c + 31 of the region (channels past cin read as zero)
+        for (int it = threadIdx.x; it < nreg * 4; it += kConvThreads) {
+            const int p = it >> 2, q = it & 3;
+            const int ry = p / RW, rx = p - ry * RW;
+            const int gy = wrap(y0 + ry - 1, H), gx = wrap(x0 + rx - 1, W);
+            const int ch = (c << 5) + (q << 3);
+            u32x4 v{0u, 0u, 0u, 0u};
+            if (ch < cin) {
+                const float4* src = reinterpret_cast<const float4*>(x + ((size_t(b) * H + gy) * W + gx) * cin + ch);
+                const float4 lo = src[0], hi = src[1];
+                v = u32x4{pack_bf16(lo.x, lo.y), pack_bf16(lo.z, lo.w), pack_bf16(hi.x, hi.y), pack_bf16(hi.z, hi.w)};
+            }
+            *reinterpret_cast<u32x4*>(s_in + rec_at(p, q)) = v;
+        }
+        __syncthreads();
+#pragma unroll 3
+        for (int t = 0; t < 9; ++t) {
+            const int toff = (t / 3) * RW + (t % 3);
+            bf16x8 bv[kMaxBlocksPerWave];
 #pragma unroll
-    for (int c = 0; c < NCOT; ++c) {
-        const int co = c * 16 + m;
-        const float bb = bias ? bias[co] : 0.f;
+            for (int i = 0; i < kMaxBlocksPerWave; ++i)
+                if (wave + kConvWaves * i < nmb)
+                    bv[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(s_in + rec_at(rp0[i] + toff, g)));
+            const u32x4* wt = wp + (size_t(t * nchunk + c) * NT) * 64 + lane;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const size_t pix = (size_t(b) * H + row) * W + x0 + kq * 4 + i;
-            float v = acc[c][i] + bb;
-            if (pre) pre[pix * cout + co] = v;
-            if (epi == kGelu) v = gelu(v);
-            else if (epi == kGeluGrad) v *= gelu_grad(gpre[pix * cout + co]);
-            if (co < cout_store) y[pix * cout_store + co] = v;
+            for (int nt = 0; nt < NT; ++nt) {
+                const bf16x8 av = __builtin_bit_cast(bf16x8, wt[nt * 64]);
+#pragma unroll
+                for (int i = 0; i < kMaxBlocksPerWave; ++i)
+                    if (wave + kConvWaves * i < nmb)
+                        acc[i][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[i], acc[i][nt], 0, 0, 0);
+            }
+        }
+        __syncthreads();  // the next chunk overwrites the region
+    }
+
+    // lane (m, g) holds output channels 16 nt + 4g .. + 3 of tile pixel 16 mb + m
+    const int cout = NT * 16;
+#pragma unroll
+    for (int i = 0; i < kMaxBlocksPerWave; ++i) {
+        const int mb = wave + kConvWaves * i;
+        if (mb >= nmb) continue;
+        const int pix = mb * 16 + m;
+        const int py = pix / TW, px = pix - py * TW;
+        const size_t gp = (size_t(b) * H + y0 + py) * W + x0 + px;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int co = nt * 16 + 4 * g;
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float z = acc[i][nt][e] + (bias ? bias[co + e] : 0.f);
+                if (pre) pre[gp * cout + co + e] = z;
+                if (epi == kGelu) z = gelu(z);
+                else if (epi == kGeluGrad) z *= gelu_grad(gpre[gp * cout + co + e]);
+                v[e] = z;
+            }
+            if ((cout_store & 3) == 0 && co + 4 <= cout_store) {
+                *reinterpret_cast<float4*>(y + gp * cout_store + co) = float4{v[0], v[1], v[2], v[3]};
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (co + e < cout_store) y[gp * cout_store + co + e] = v[e];
+            }
         }
     }
 }
 
-// Weight gradient partials: one wavefront per (band of `rows` image rows, tap).  dW[t][ci][co] =
-// sum over pixels p of x[p + off(t)][ci] * g[p][co]; the centre-tap wavefront also sums g (bias grad).
-template <int NC16, int NCOT>
+// Weight gradient partials: one wavefront per (band of `rows` image rows, tap, 16 input channels).
+// dW[t][ci][co] = sum over the band's pixels p of x[p + off(t)][ci] * g[p][co], 16 pixels at a time in
+// row-major order within the band (rows * W a multiple of 16); the (centre tap, first channel tile)
+// wavefront also sums g over the band (bias gradient).  M = 16 input channels, N = 16 output channels,
+// K = 16 pixels per v_mfma_f32_16x16x16_bf16.
+template <int NCOT>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ x, const float* __restrict__ g,
-                                                    int B, int H, int W, int rows, float* __restrict__ part,
+                                                    int B, int H, int W, int cin, int rows, float* __restrict__ part,
                                                     float* __restrict__ bpart) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int nc16 = cin >> 4;
     const int bands_per_img = H / rows;
     const int nbands = B * bands_per_img;
-    if (wave >= nbands * 9) return;
-    const int t = wave % 9;
-    const int band = wave / 9;
+    if (wave >= nbands * 9 * nc16) return;
+    const int ct = wave % nc16;
+    const int t = (wave / nc16) % 9;
+    const int band = wave / (nc16 * 9);
     const int b = band / bands_per_img, r0 = (band % bands_per_img) * rows;
-    const int Cin = NC16 * 16, Cout = NCOT * 16;
+    const int Cout = NCOT * 16;
     const int m = lane & 15, kq = lane >> 4;
     const int dy = t / 3 - 1, dx = t % 3 - 1;
-    v4f acc[NC16][NCOT];
-    for (int i = 0; i < NC16; ++i)
-        for (int j = 0; j < NCOT; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    const int ci = ct * 16 + m;
+    const bool want_bias = t == 4 && ct == 0;
+    v4f acc[NCOT];
     float bsum[NCOT];
-    for (int j = 0; j < NCOT; ++j) bsum[j] = 0.f;
-    for (int r = r0; r < r0 + rows; ++r) {
-        const int ys = wrap(r + dy, H);
-        for (int x0 = 0; x0 < W; x0 += 16) {
-            // this lane's 4 pixels (k index 4*kq + i) of the 16-pixel K block
-            size_t ps[4], pd[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int xd = x0 + kq * 4 + i;
-                pd[i] = (size_t(b) * H + r) * W + xd;
-                ps[i] = (size_t(b) * H + ys) * W + wrap(xd + dx, W);
-            }
+    for (int j = 0; j < NCOT; ++j) {
+        acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
+        bsum[j] = 0.f;
+    }
+    const int npix = rows * W;
+    for (int p0 = 0; p0 < npix; p0 += 16) {
+        // this lane's 4 pixels (k index 4 kq + i) of the 16-pixel K block
+        size_t ps[4], pd[4];
 #pragma unroll
-            for (int j = 0; j < NCOT; ++j) {
-                const int co = j * 16 + m;
-                const float g0 = g[pd[0] * Cout + co], g1 = g[pd[1] * Cout + co];
-                const float g2 = g[pd[2] * Cout + co], g3 = g[pd[3] * Cout + co];
-                const v4s bv = to_bf16x4(g0, g1, g2, g3);
-                bsum[j] += (g0 + g1) + (g2 + g3);
+        for (int i = 0; i < 4; ++i) {
+            const int p = p0 + kq * 4 + i;
+            const int r = r0 + p / W, xd = p % W;
+            pd[i] = (size_t(b) * H + r) * W + xd;
+            ps[i] = (size_t(b) * H + wrap(r + dy, H)) * W + wrap(xd + dx, W);
+        }
+        const v4s av = to_bf16x4(x[ps[0] * cin + ci], x[ps[1] * cin + ci], x[ps[2] * cin + ci], x[ps[3] * cin + ci]);
 #pragma unroll
-                for (int i = 0; i < NC16; ++i) {
-                    const int ci = i * 16 + m;
-                    const v4s av = to_bf16x4(x[ps[0] * Cin + ci], x[ps[1] * Cin + ci], x[ps[2] * Cin + ci],
-                                             x[ps[3] * Cin + ci]);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[i][j], 0, 0, 0);
-                }
-            }
+        for (int j = 0; j < NCOT; ++j) {
+            const int co = j * 16 + m;
+            const float g0 = g[pd[0] * Cout + co], g1 = g[pd[1] * Cout + co];
+            const float g2 = g[pd[2] * Cout + co], g3 = g[pd[3] * Cout + co];
+            if (want_bias) bsum[j] += (g0 + g1) + (g2 + g3);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, to_bf16x4(g0, g1, g2, g3), acc[j], 0, 0, 0);
         }
     }
-    float* out = part + (size_t(band) * 9 + t) * Cin * Cout;
+    float* out = part + (size_t(band) * 9 + t) * cin * Cout;
 #pragma unroll
-    for (int i = 0; i < NC16; ++i)
+    for (int j = 0; j < NCOT; ++j)
 #pragma unroll
-        for (int j = 0; j < NCOT; ++j)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) out[(i * 16 + kq * 4 + e) * Cout + j * 16 + m] = acc[i][j][e];
-    if (t == 4) {  // bias gradient of this band: sum the 4 lane groups holding the same channel
+        for (int e = 0; e < 4; ++e) out[(ct * 16 + kq * 4 + e) * Cout + j * 16 + m] = acc[j][e];
+    if (want_bias) {  // bias gradient of this band: sum the 4 lane groups holding the same channel
 #pragma unroll
         for (int j = 0; j < NCOT; ++j) {
             float s = bsum[j];
@@ -208,42 +293,60 @@ using namespace npde_nn;
 
 extern "C" {
 
-// Forward / data-gradient convolution (see the file header).  cout = 16 or 32; cin a multiple of 16.
+// Forward / data-gradient convolution (see the file header).  cin a multiple of 8 (x 16-byte aligned);
+// cout 16 .. 128 or 192 in steps of 16; W a multiple of 8; H a multiple of the tile's rows (8 or 16, or
+// H itself when smaller).
 int npde_conv3x3(const float* x, int B, int H, int W, int cin, const void* wpack, const float* bias, int cout,
                  int cout_store, float* y, float* pre, int epilogue, const float* gpre, void* stream) {
-    if (!x || !wpack || !y || B <= 0 || H <= 0 || W <= 0 || (W % 16) || (cin % 16) || cin <= 0 ||
-        cout_store <= 0 || cout_store > cout || (epilogue == kGeluGrad && !gpre))
+    if (!x || !wpack || !y || B <= 0 || H <= 0 || W <= 0 || (W % 8) || (cin % 8) || cin <= 0 ||
+        (reinterpret_cast<uintptr_t>(x) & 15) || cout_store <= 0 || cout_store > cout ||
+        (epilogue == kGeluGrad && !gpre))
         return 1;
-    const int waves = B * H * (W / 16);
-    const dim3 grid((waves + 3) / 4), block(256);
+    // tile: 16 columns when the width allows, else 8; as many rows as divide H up to 128 pixels
+    const int TW = W % 16 == 0 ? 16 : 8;
+    int TH = kTilePixels / TW;
+    while (TH > 1 && (H % TH != 0 || (TH * TW) % 16 != 0)) --TH;
+    if ((TH * TW) % 16 != 0 || H % TH != 0) return 1;
+    const dim3 grid(unsigned(B * (H / TH) * (W / TW))), block(kConvThreads);
     hipStream_t s = (hipStream_t)stream;
-    const v4s* wp = (const v4s*)wpack;
+    const u32x4* wp = (const u32x4*)wpack;
+#define NPDE_CONV_CASE(NT)                                                                                   \
+    case NT * 16:                                                                                            \
+        hipLaunchKernelGGL(k_conv_fwd<NT>, grid, block, 0, s, x, B, H, W, cin, TH, TW, wp, bias, cout_store, y, \
+                           pre, epilogue, gpre);                                                             \
+        break;
     switch (cout) {
-        case 16: hipLaunchKernelGGL(k_conv_fwd<1>, grid, block, 0, s, x, B, H, W, cin, wp, bias, cout_store, y, pre, epilogue, gpre); break;
-        case 32: hipLaunchKernelGGL(k_conv_fwd<2>, grid, block, 0, s, x, B, H, W, cin, wp, bias, cout_store, y, pre, epilogue, gpre); break;
-        case 48: hipLaunchKernelGGL(k_conv_fwd<3>, grid, block, 0, s, x, B, H, W, cin, wp, bias, cout_store, y, pre, epilogue, gpre); break;
-        case 64: hipLaunchKernelGGL(k_conv_fwd<4>, grid, block, 0, s, x, B, H, W, cin, wp, bias, cout_store, y, pre, epilogue, gpre); break;
+        NPDE_CONV_CASE(1)
+        NPDE_CONV_CASE(2)
+        NPDE_CONV_CASE(3)
+        NPDE_CONV_CASE(4)
+        NPDE_CONV_CASE(5)
+        NPDE_CONV_CASE(6)
+        NPDE_CONV_CASE(7)
+        NPDE_CONV_CASE(8)
+        NPDE_CONV_CASE(12)
         default: return 1;
     }
+#undef NPDE_CONV_CASE
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
-// Weight and bias gradients.  part must hold (B*H/rows) * 9 * cin * cout floats and bpart
+// Weight and bias gradients.  cin a multiple of 16 up to 256, cout a multiple of 16 up to 128; rows divides
+// H and rows * W is a multiple of 16.  part must hold (B*H/rows) * 9 * cin * cout floats and bpart
 // (B*H/rows) * cout; dw is written as [tap][ci][co], db as [co].
 int npde_conv3x3_wgrad(const float* x, const float* g, int B, int H, int W, int cin, int cout, int rows,
                        float* part, float* bpart, float* dw, float* db, void* stream) {
-    if (!x || !g || !part || !bpart || !dw || !db || rows <= 0 || (H % rows) || (W % 16) || B <= 0) return 1;
+    if (!x || !g || !part || !bpart || !dw || !db || rows <= 0 || B <= 0 || H <= 0 || W <= 0 || (H % rows) ||
+        ((rows * W) % 16) || cin <= 0 || cout <= 0 || (cin % 16) || (cout % 16) || cin > 256 || cout > 128)
+        return 1;
     const int nbands = B * (H / rows);
-    const dim3 grid((nbands * 9 + 3) / 4), block(256);
+    const int waves = nbands * 9 * (cin / 16);
+    const dim3 grid(unsigned((waves + 3) / 4)), block(256);
     hipStream_t s = (hipStream_t)stream;
-    if (cin <= 0 || cout <= 0 || (cin % 16) || (cout % 16) || cin > 64 || cout > 64) return 1;
-    using WgradFn = void (*)(const float*, const float*, int, int, int, int, float*, float*);
-    static constexpr WgradFn table[4][4] = {
-        {k_conv_wgrad<1, 1>, k_conv_wgrad<1, 2>, k_conv_wgrad<1, 3>, k_conv_wgrad<1, 4>},
-        {k_conv_wgrad<2, 1>, k_conv_wgrad<2, 2>, k_conv_wgrad<2, 3>, k_conv_wgrad<2, 4>},
-        {k_conv_wgrad<3, 1>, k_conv_wgrad<3, 2>, k_conv_wgrad<3, 3>, k_conv_wgrad<3, 4>},
-        {k_conv_wgrad<4, 1>, k_conv_wgrad<4, 2>, k_conv_wgrad<4, 3>, k_conv_wgrad<4, 4>}};
-    hipLaunchKernelGGL(table[cin / 16 - 1][cout / 16 - 1], grid, block, 0, s, x, g, B, H, W, rows, part, bpart);
+    using WgradFn = void (*)(const float*, const float*, int, int, int, int, int, float*, float*);
+    static constexpr WgradFn table[8] = {k_conv_wgrad<1>, k_conv_wgrad<2>, k_conv_wgrad<3>, k_conv_wgrad<4>,
+                                         k_conv_wgrad<5>, k_conv_wgrad<6>, k_conv_wgrad<7>, k_conv_wgrad<8>};
+    hipLaunchKernelGGL(table[cout / 16 - 1], grid, block, 0, s, x, g, B, H, W, cin, rows, part, bpart);
     const int nw = 9 * cin * cout;
     hipLaunchKernelGGL(k_wgrad_reduce, dim3((nw + cout + 255) / 256), dim3(256), 0, s, part, bpart, nbands, nw, cout,
                        dw, db);

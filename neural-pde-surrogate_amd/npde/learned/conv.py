@@ -8,8 +8,8 @@ its backward runs, per layer from the top, npde_conv3x3_wgrad (weight and bias g
 data-gradient convolution with GELU'(z_{k-1}) fused into its epilogue.
 
 Tensors: activations NHWC float32 on the GPU with channels padded to a multiple of 16 (values enter
-the MFMAs rounded to bfloat16); weights (Cout, Cin, 3, 3) float32 master copies, packed to BF16
-MFMA operand order on every call.
+the MFMAs rounded to bfloat16: the forward kernel stages each input tile in LDS as bfloat16 once);
+weights (Cout, Cin, 3, 3) float32 master copies, packed to BF16 MFMA operand order on every call.
 """
 from __future__ import annotations
 
@@ -28,19 +28,19 @@ def _pad16(c: int) -> int:
 
 
 def pack_weights(w: torch.Tensor, transpose_flip: bool = False) -> torch.Tensor:
-    """(Cout, Cin, 3, 3) float32 -> bfloat16 [tap][ci/16][co/16][lane][4] with B[k][n] = W[ci][co] where
-    lane = 4*(k/4)... i.e. element (lane = kq*16 + m, i) holds ci = c16*16 + 4*kq + i, co = cot*16 + m.
+    """(Cout, Cin, 3, 3) float32 -> bfloat16 A-operand fragments of the forward kernel,
+    [tap][ci/32][co/16][lane][8]: lane = 16 g + m, element j holds W[co = 16 t + m][ci = 32 c + 8 g + j][tap]
+    (tap = ky * 3 + kx), zero beyond Cin (padded to a multiple of 32) and Cout (to a multiple of 16).
     transpose_flip packs the data-gradient operator (taps reversed, ci and co exchanged)."""
-    cout, cin = w.shape[:2]
-    ci_p, co_p = _pad16(cin), _pad16(cout)
-    wp = w.new_zeros(co_p, ci_p, 3, 3)
-    wp[:cout, :cin] = w
-    wt = wp.permute(2, 3, 1, 0).reshape(9, ci_p, co_p)  # [tap = ky*3 + kx][ci][co]
+    wt = w.permute(2, 3, 1, 0).reshape(9, w.shape[1], w.shape[0])  # [tap][ci][co]
     if transpose_flip:
         wt = wt.flip(0).transpose(1, 2)
     n_in, n_out = wt.shape[1], wt.shape[2]
-    wt = wt.reshape(9, n_in // 16, 4, 4, n_out // 16, 16).permute(0, 1, 4, 2, 5, 3)
-    return wt.contiguous().to(torch.bfloat16).contiguous()
+    ci_p, co_p = (n_in + 31) // 32 * 32, _pad16(n_out)
+    wp = wt.new_zeros(9, ci_p, co_p)
+    wp[:, :n_in, :n_out] = wt
+    wp = wp.reshape(9, ci_p // 32, 4, 8, co_p // 16, 16).permute(0, 1, 4, 2, 5, 3)  # [tap][c][t][g][m][j]
+    return wp.contiguous().to(torch.bfloat16).contiguous()
 
 
 def _ptr(t):

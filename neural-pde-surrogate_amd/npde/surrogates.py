@@ -13,11 +13,13 @@ The architectures of the reference's surrogate/models.py (Equinox), rebuilt on t
                                are positive by construction.
 
 All convolutions use periodic padding, so the ResNet and FNO are exactly translation-equivariant on the
-torus (the U-Net for shifts that are multiples of 2**levels).  ResNet(backend="mfma") runs its k = 3
-convolutions on the library's BF16-MFMA periodic convolution kernels (npde_conv3x3 and its weight
-gradient; forward and backward) in NHWC, with the GELU that follows the first convolution of each block in
-the kernel's epilogue; the 1x1 lift and projection are matrix products.  Its parameters are the torch
-backend's, so a model can switch backends.  GELU is the tanh approximation (JAX's
+torus (the U-Net for shifts that are multiples of 2**levels).  ResNet(backend="mfma") and UNet(backend="mfma")
+run their 3x3 convolutions on the library's BF16-MFMA periodic convolution kernels (npde_conv3x3 and its
+weight gradient; forward and backward) in NHWC, with the GELU that follows the first convolution of each
+block in the kernel's epilogue; the 1x1 convolutions are matrix products and the U-Net's pooling,
+upsampling and skip concatenation tensor operations on the NHWC activations.  FNO(backend="dft") computes
+its spectral layers as truncated discrete Fourier transforms in matrix products (only the kept modes are
+formed) instead of rfft2 / irfft2.  Each backend uses the torch backend's parameters, so a model can switch.  GELU is the tanh approximation (JAX's
 default, which the reference uses).  Parameter shapes follow the reference, so parameter counts agree:
 ResNet(64, 8) 591,428; UNet(16, 3) 516,516; FNO(32, 4, 16 modes) 4,203,428.
 """
@@ -138,9 +140,9 @@ class ResNet(torch.nn.Module):
         BF16 products, FP32 accumulation)."""
         width = self.lift.weight.shape[0]
         H, W = x.shape[-2:]
-        if self.kernel_size != 3 or width % 16 or width > 64 or W % 16 or H % 8 or x.dtype != torch.float32:
-            raise ValueError("the MFMA backend needs k = 3, width a multiple of 16 up to 64, W a multiple of 16, "
-                             "H a multiple of 8 and float32 input")
+        if self.kernel_size != 3 or width % 16 or width > 128 or W % 8 or H % 8 or x.dtype != torch.float32:
+            raise ValueError("the MFMA backend needs k = 3, width a multiple of 16 up to 128, H and W multiples of "
+                             "8 and float32 input")
         h = _conv1x1_nhwc(x.permute(0, 2, 3, 1), self.lift)
         for b in self.blocks:
             t = Conv3x3MFMA.apply(gelu(h), b.conv1.weight, b.conv1.bias, True)
@@ -175,7 +177,7 @@ def upsample(x):
 
 
 class UNet(torch.nn.Module):
-    def __init__(self, width=16, levels=3, delta_scale=None, channels=4, seed=0):
+    def __init__(self, width=16, levels=3, delta_scale=None, channels=4, seed=0, backend="torch"):
         super().__init__()
         g = torch.Generator().manual_seed(seed)
         ch = [width * 2 ** l for l in range(levels + 1)]
@@ -185,8 +187,47 @@ class UNet(torch.nn.Module):
         self.up = torch.nn.ModuleList([ConvBlock(ch[l + 1] + ch[l], ch[l], g) for l in range(levels)])
         self.project = PeriodicConv(ch[0], channels, 1, g).zero_()
         self.register_buffer("delta_scale", _scale(delta_scale, channels))
+        self.levels = levels
+        self.backend = backend
+
+    def _forward_mfma(self, x):
+        """The same network in NHWC with every 3x3 convolution on the MFMA kernels (float32 activations, BF16
+        products, FP32 accumulation); pooling, upsampling, concatenation and the 1x1 convolutions are tensor
+        operations on the NHWC activations."""
+        H, W = x.shape[-2:]
+        widths = [self.lift.weight.shape[0] * 2 ** l for l in range(self.levels + 1)]
+        if (any(c % 16 for c in widths) or widths[-1] > 128 or H % (8 << self.levels) or W % (8 << self.levels)
+                or x.dtype != torch.float32):
+            raise ValueError("the MFMA backend needs widths that are multiples of 16 up to 128 at the bottleneck, "
+                             "H and W multiples of 8 * 2**levels and float32 input")
+
+        def block(b, h):
+            t = Conv3x3MFMA.apply(gelu(h), b.conv1.weight, b.conv1.bias, True)
+            out = Conv3x3MFMA.apply(t, b.conv2.weight, b.conv2.bias, False)
+            return (h if b.skip is None else _conv1x1_nhwc(h, b.skip)) + out
+
+        def down(h):  # 2x2 average pooling
+            B, n, m, c = h.shape
+            return h.reshape(B, n // 2, 2, m // 2, 2, c).mean(dim=(2, 4))
+
+        def up(h):  # nearest-neighbour upsampling by 2
+            return h.repeat_interleave(2, dim=1).repeat_interleave(2, dim=2)
+
+        h = _conv1x1_nhwc(x.permute(0, 2, 3, 1), self.lift)
+        skips = []
+        for b in self.down:
+            h = block(b, h)
+            skips.append(h)
+            h = down(h)
+        h = block(self.bottleneck, h)
+        for b, skip in zip(reversed(self.up), reversed(skips)):
+            h = block(b, torch.cat([up(h), skip], dim=-1))
+        out = _conv1x1_nhwc(gelu(h), self.project).permute(0, 3, 1, 2)
+        return x + self.delta_scale.view(1, -1, 1, 1).to(x.dtype) * out
 
     def forward(self, x):
+        if self.backend == "mfma":
+            return self._forward_mfma(x)
         h = self.lift(x)
         skips = []
         for block in self.down:
@@ -210,8 +251,62 @@ class SpectralConv2d(torch.nn.Module):
         self.w_re = torch.nn.Parameter(scale * torch.rand(shape, generator=generator))
         self.w_im = torch.nn.Parameter(scale * torch.rand(shape, generator=generator))
         self.modes = modes
+        self.backend = "torch"
+        self._bases = {}
+
+    def _basis(self, n, device, dtype):
+        """cos / sin tables of the kept modes: columns ky in [0, m) for the real transform along the last axis,
+        rows kx in [0, m) and [n - m, n) for the complex one along the other, in float64, cast once."""
+        key = (n, device, dtype)
+        if key not in self._bases:
+            m = self.modes
+            r = torch.arange(n, dtype=torch.float64)
+            ky = torch.arange(m, dtype=torch.float64)
+            kx = torch.cat([torch.arange(m), torch.arange(n - m, n)]).to(torch.float64)
+            ay = 2 * math.pi * torch.outer(r, ky) / n   # (n, m)
+            ax = 2 * math.pi * torch.outer(kx, r) / n   # (2m, n)
+            cy = torch.full((m,), 2.0, dtype=torch.float64)
+            cy[0] = 1.0                                 # one-sided spectrum: ky > 0 stands for +-ky
+            t = {"cy": torch.cos(ay), "sy": torch.sin(ay), "cx": torch.cos(ax), "sx": torch.sin(ax),
+                 "icy": (cy[:, None] * torch.cos(ay).t()) / (n * n), "isy": (cy[:, None] * torch.sin(ay).t()) / (n * n)}
+            self._bases[key] = {k: v.to(device=device, dtype=dtype).contiguous() for k, v in t.items()}
+        return self._bases[key]
+
+    def _forward_dft(self, x):
+        """The layer as a truncated discrete Fourier transform in matrix products (rocBLAS/hipBLASLt GEMMs):
+        only the 2m x m kept modes are ever formed.  Forward along the last axis (real input, ky < m), then
+        along the other (kx in [0, m) and [n - m, n)); the per-mode channel mixing as one batched product over
+        the modes; back along the first axis, then the real inverse along the last, where a mode ky > 0 counts
+        twice and the imaginary part of ky = 0 drops out (torch.fft.irfft2's one-sided convention).  Equal to
+        the rfft2 path to rounding."""
+        B, c_in, n, n2 = x.shape
+        m = self.modes
+        if n2 != n or 2 * m > n:
+            raise ValueError("the DFT path needs square inputs and 2 * modes <= n")
+        T = self._basis(n, x.device, x.dtype)
+        ar, ai = x @ T["cy"], -(x @ T["sy"])                        # (B, ci, n, m): sum_y x e^{-i ky y}
+        xr = torch.matmul(T["cx"], ar) + torch.matmul(T["sx"], ai)  # (B, ci, 2m, m): sum_x (.) e^{-i kx x}
+        xi = torch.matmul(T["cx"], ai) - torch.matmul(T["sx"], ar)
+        wr = torch.cat([self.w_re[0], self.w_re[1]], dim=2)        # (ci, co, 2m, m): kx in [0, m), then [n-m, n)
+        wi = torch.cat([self.w_im[0], self.w_im[1]], dim=2)
+        # per mode (kx, ky): Y[b, o] = sum_i X[b, i] W[i, o], complex; modes as the batch of one product
+        Xr = xr.permute(2, 3, 0, 1).reshape(2 * m * m, B, c_in)
+        Xi = xi.permute(2, 3, 0, 1).reshape(2 * m * m, B, c_in)
+        Wr = wr.permute(2, 3, 0, 1).reshape(2 * m * m, c_in, -1).to(x.dtype)
+        Wi = wi.permute(2, 3, 0, 1).reshape(2 * m * m, c_in, -1).to(x.dtype)
+        yr = torch.bmm(Xr, Wr) - torch.bmm(Xi, Wi)
+        yi = torch.bmm(Xr, Wi) + torch.bmm(Xi, Wr)
+        c_out = yr.shape[-1]
+        yr = yr.reshape(2 * m, m, B, c_out).permute(2, 3, 0, 1)   # (B, co, 2m, m)
+        yi = yi.reshape(2 * m, m, B, c_out).permute(2, 3, 0, 1)
+        # back along the first axis: sum_kx Y e^{+i kx x} (n, 2m) @ (2m, m); then the real inverse along the last
+        zr = torch.matmul(T["cx"].t(), yr) - torch.matmul(T["sx"].t(), yi)  # (B, co, n, m)
+        zi = torch.matmul(T["cx"].t(), yi) + torch.matmul(T["sx"].t(), yr)
+        return zr @ T["icy"] - zi @ T["isy"]                        # (B, co, n, n)
 
     def forward(self, x):
+        if self.backend == "dft":
+            return self._forward_dft(x)
         B, c_in, n, _ = x.shape
         m = self.modes
         X = torch.fft.rfft2(x)
@@ -233,11 +328,19 @@ class FNOBlock(torch.nn.Module):
 
 
 class FNO(torch.nn.Module):
-    def __init__(self, width=32, depth=4, modes=12, delta_scale=None, channels=4, hidden=128, seed=0):
+    """backend "torch": spectral layers on torch.fft (rocFFT); "dft": on the truncated-DFT matrix products of
+    SpectralConv2d._forward_dft.  The same parameters serve both."""
+
+    def __init__(self, width=32, depth=4, modes=12, delta_scale=None, channels=4, hidden=128, seed=0,
+                 backend="torch"):
         super().__init__()
         g = torch.Generator().manual_seed(seed)
         self.lift = PeriodicConv(channels, width, 1, g)
         self.blocks = torch.nn.ModuleList([FNOBlock(width, modes, g) for _ in range(depth)])
+        if backend not in ("torch", "dft"):
+            raise ValueError("FNO backend must be 'torch' or 'dft'")
+        for b in self.blocks:
+            b.spectral.backend = backend
         self.project1 = PeriodicConv(width, hidden, 1, g)
         self.project2 = PeriodicConv(hidden, channels, 1, g).zero_()
         self.register_buffer("delta_scale", _scale(delta_scale, channels))

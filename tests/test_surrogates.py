@@ -87,6 +87,33 @@ def test_spectral_conv_keeps_only_low_modes_and_is_linear():
     assert torch.allclose(layer(2 * a - 3 * b), 2 * layer(a) - 3 * layer(b), atol=1e-12)
 
 
+@pytest.mark.parametrize("n,m", [(16, 4), (16, 8), (32, 12), (64, 16)])
+def test_spectral_conv_dft_path_equals_rfft2(n, m):
+    """The truncated-DFT path (backend "dft": matrix products over the kept modes only) equals the rfft2 /
+    irfft2 path to binary64 rounding, in the output and in the gradients of the input and of both weight
+    blocks; including 2 m = n, where the two kx blocks meet.  A sign, conjugation or normalisation slip in
+    either transform, or a swapped weight block, fails by O(1)."""
+    torch.manual_seed(n + m)
+    layer = M.SpectralConv2d(3, 5, m, torch.Generator().manual_seed(1)).double()
+    x = torch.randn(2, 3, n, n, dtype=torch.float64, requires_grad=True)
+    res = {}
+    for backend in ("torch", "dft"):
+        layer.backend = backend
+        y = layer(x)
+        res[backend] = (y.detach(), torch.autograd.grad((y * torch.cos(y)).sum(), [x, layer.w_re, layer.w_im]))
+    (ya, ga), (yb, gb) = res["torch"], res["dft"]
+    assert float((ya - yb).abs().max()) < 1e-12 * float(ya.abs().max())
+    for u, v in zip(ga, gb):  # sums over n^2 terms: 1e-11 of the largest entry
+        assert float((u - v).abs().max()) < 1e-11 * float(u.abs().max())
+    fno_t = _noisy(M.FNO(width=8, depth=2, modes=m // 2 or 1)).double()
+    fno_d = M.FNO(width=8, depth=2, modes=m // 2 or 1, backend="dft").double()
+    fno_d.load_state_dict(fno_t.state_dict())
+    z = torch.randn(1, 4, n, n, dtype=torch.float64)
+    assert torch.allclose(fno_t(z), fno_d(z), rtol=0, atol=1e-12)
+    with pytest.raises(ValueError):
+        M.FNO(width=8, depth=1, modes=4, backend="fft")
+
+
 def test_fno_runs_at_any_resolution():
     m = _noisy(M.FNO(width=8, depth=2, modes=4)).double()
     n1 = torch.linspace(0, 2 * math.pi, 33, dtype=torch.float64)[:-1]
@@ -225,7 +252,70 @@ def test_resnet_on_the_mfma_kernels_matches_the_torch_backend(npde_mod):
         losses.append(float(loss))
     assert losses[-1] < 0.9 * losses[0], losses
     with pytest.raises(ValueError):
-        M.ResNet(width=128, depth=1, backend="mfma").to(dev)(x)
+        M.ResNet(width=144, depth=1, backend="mfma").to(dev)(x)
+
+
+@pytest.mark.gpu
+def test_unet_on_the_mfma_kernels_matches_the_torch_backend(npde_mod):
+    """UNet(backend="mfma") -- every 3x3 convolution on the library's BF16-MFMA kernels, forward and backward,
+    with input channels 16 to 192 (skip concatenations), output channels 16 to 128 and images down to 8 x 8 --
+    against the same parameters on torch's float32 convolutions: output change and every parameter gradient
+    within BF16 tolerance (relative L2 error below 3e-2 and 6e-2), at the reference's default UNet(16, 3) and a
+    two-level width-32 variant; and a few Adam steps on the MFMA backend lower the one-batch loss."""
+    dev = torch.device("cuda")
+    for width, levels in ((16, 3), (32, 2)):
+        m = M.UNet(width=width, levels=levels, seed=5).to(dev)
+        with torch.no_grad():
+            g = torch.Generator().manual_seed(1)
+            m.project.weight.copy_(torch.randn(m.project.weight.shape, generator=g) * 0.1)
+            m.project.bias.copy_(torch.randn(m.project.bias.shape, generator=g) * 0.1)
+        x = torch.randn(3, 4, 64, 64, generator=torch.Generator().manual_seed(2)).to(dev)
+        out, grads = {}, {}
+        for backend in ("torch", "mfma"):
+            m.backend = backend
+            m.zero_grad()
+            y = m(x)
+            (y ** 2).mean().backward()
+            out[backend] = (y - x).detach()
+            grads[backend] = {k: p.grad.detach().clone() for k, p in m.named_parameters()}
+        rel = lambda a, b: float((a - b).norm() / b.norm())
+        assert rel(out["mfma"], out["torch"]) < 3e-2, (width, levels)
+        for k in grads["torch"]:
+            assert rel(grads["mfma"][k], grads["torch"][k]) < 6e-2, (width, levels, k)
+    m.backend = "mfma"
+    y_t = (x + 0.1 * torch.randn(x.shape, generator=torch.Generator().manual_seed(4)).to(dev)).detach()
+    opt = torch.optim.Adam(m.parameters(), lr=1e-3)
+    losses = []
+    for _ in range(8):
+        opt.zero_grad()
+        loss = ((m(x) - y_t) ** 2).mean()
+        loss.backward()
+        opt.step()
+        losses.append(float(loss))
+    assert losses[-1] < 0.9 * losses[0], losses
+    with pytest.raises(ValueError):  # 96 x 96 does not halve to a multiple of 8 three times
+        M.UNet(width=16, levels=3, backend="mfma").to(dev)(torch.zeros(1, 4, 96, 96, device=dev))
+
+
+@pytest.mark.gpu
+def test_fno_dft_path_on_the_gpu_matches_rfft2(npde_mod):
+    """The FNO's truncated-DFT spectral layers on the GPU (float32 matrix products) against its rocFFT layers
+    with the same parameters: outputs and gradients within float32 rounding (relative L2 below 1e-4)."""
+    dev = torch.device("cuda")
+    m = _noisy(M.FNO(width=32, depth=4, modes=16)).to(dev)
+    x = torch.randn(4, 4, 64, 64, generator=torch.Generator().manual_seed(3)).to(dev)
+    res = {}
+    for backend in ("torch", "dft"):
+        for b in m.blocks:
+            b.spectral.backend = backend
+        m.zero_grad()
+        y = m(x)
+        (y ** 2).mean().backward()
+        res[backend] = (y.detach(), {k: p.grad.detach().clone() for k, p in m.named_parameters()})
+    rel = lambda a, b: float((a - b).norm() / b.norm())
+    assert rel(res["dft"][0], res["torch"][0]) < 1e-4
+    for k in res["torch"][1]:
+        assert rel(res["dft"][1][k], res["torch"][1][k]) < 1e-4, k
 
 
 def test_rollout_drift_and_bad_cells_hand_computed():
