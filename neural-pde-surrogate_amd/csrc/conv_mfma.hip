@@ -186,63 +186,85 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const float* __restri
     }
 }
 
-// Weight gradient partials: one wavefront per (band of `rows` image rows, tap, 16 input channels).
-// dW[t][ci][co] = sum over the band's pixels p of x[p + off(t)][ci] * g[p][co], 16 pixels at a time in
-// row-major order within the band (rows * W a multiple of 16); the (centre tap, first channel tile)
+// Weight gradient partials: one wavefront per (band of `rows` image rows, tap, group of NCI x 16 input
+// channels; NCI = 4 when cin is a multiple of 64, else 2 or 1).  dW[t][ci][co] = sum over the band's pixels p of x[p + off(t)][ci] * g[p][co], 16 pixels
+// at a time in row-major order within the band (rows * W a multiple of 16); the (centre tap, first group)
 // wavefront also sums g over the band (bias gradient).  M = 16 input channels, N = 16 output channels,
-// K = 16 pixels per v_mfma_f32_16x16x16_bf16.
-template <int NCOT>
+// K = 16 pixels per v_mfma_f32_16x16x16_bf16; the gradient values of a K block are loaded once for all the
+// group's channel tiles.
+template <int NCI, int NCOT>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ x, const float* __restrict__ g,
                                                     int B, int H, int W, int cin, int rows, float* __restrict__ part,
                                                     float* __restrict__ bpart) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int nc16 = cin >> 4;
+    const int ngroups = nc16 / NCI;
     const int bands_per_img = H / rows;
     const int nbands = B * bands_per_img;
-    if (wave >= nbands * 9 * nc16) return;
-    const int ct = wave % nc16;
-    const int t = (wave / nc16) % 9;
-    const int band = wave / (nc16 * 9);
+    if (wave >= nbands * 9 * ngroups) return;
+    const int grp = wave % ngroups;
+    const int t = (wave / ngroups) % 9;
+    const int band = wave / (ngroups * 9);
     const int b = band / bands_per_img, r0 = (band % bands_per_img) * rows;
     const int Cout = NCOT * 16;
     const int m = lane & 15, kq = lane >> 4;
     const int dy = t / 3 - 1, dx = t % 3 - 1;
-    const int ci = ct * 16 + m;
-    const bool want_bias = t == 4 && ct == 0;
-    v4f acc[NCOT];
+    const int ct0 = grp * NCI;
+    const bool want_bias = t == 4 && grp == 0;
+    v4f acc[NCI][NCOT];
     float bsum[NCOT];
 #pragma unroll
     for (int j = 0; j < NCOT; ++j) {
-        acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int i = 0; i < NCI; ++i) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
         bsum[j] = 0.f;
     }
-    const int npix = rows * W;
-    for (int p0 = 0; p0 < npix; p0 += 16) {
-        // this lane's 4 pixels (k index 4 kq + i) of the 16-pixel K block
+    // the K blocks walk the band 16 pixels at a time: (rb, xb) is the block's first pixel; this lane's 4 pixels
+    // (k index 4 kq + i) sit lr rows and lx + i columns past it (W = 8: a block spans two rows)
+    const int lr = (4 * kq) / W, lx = (4 * kq) % W;
+    int rb = r0, xb = 0;
+    for (int blk = rows * W / 16; blk > 0; --blk) {
         size_t ps[4], pd[4];
+        const int r = rb + lr;
+        const size_t rd = (size_t(b) * H + r) * W, rs = (size_t(b) * H + wrap(r + dy, H)) * W;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int p = p0 + kq * 4 + i;
-            const int r = r0 + p / W, xd = p % W;
-            pd[i] = (size_t(b) * H + r) * W + xd;
-            ps[i] = (size_t(b) * H + wrap(r + dy, H)) * W + wrap(xd + dx, W);
+            const int xd = xb + lx + i;
+            pd[i] = rd + xd;
+            ps[i] = rs + wrap(xd + dx, W);
         }
-        const v4s av = to_bf16x4(x[ps[0] * cin + ci], x[ps[1] * cin + ci], x[ps[2] * cin + ci], x[ps[3] * cin + ci]);
+        xb += 16;
+        while (xb >= W) {
+            xb -= W;
+            ++rb;
+        }
+        v4s av[NCI];
+#pragma unroll
+        for (int i = 0; i < NCI; ++i) {
+            const int ci = (ct0 + i) * 16 + m;
+            av[i] = to_bf16x4(x[ps[0] * cin + ci], x[ps[1] * cin + ci], x[ps[2] * cin + ci], x[ps[3] * cin + ci]);
+        }
 #pragma unroll
         for (int j = 0; j < NCOT; ++j) {
             const int co = j * 16 + m;
             const float g0 = g[pd[0] * Cout + co], g1 = g[pd[1] * Cout + co];
             const float g2 = g[pd[2] * Cout + co], g3 = g[pd[3] * Cout + co];
             if (want_bias) bsum[j] += (g0 + g1) + (g2 + g3);
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, to_bf16x4(g0, g1, g2, g3), acc[j], 0, 0, 0);
+            const v4s bv = to_bf16x4(g0, g1, g2, g3);
+#pragma unroll
+            for (int i = 0; i < NCI; ++i)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[i], bv, acc[i][j], 0, 0, 0);
         }
     }
     float* out = part + (size_t(band) * 9 + t) * cin * Cout;
 #pragma unroll
-    for (int j = 0; j < NCOT; ++j)
+    for (int i = 0; i < NCI; ++i) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) out[(ct * 16 + kq * 4 + e) * Cout + j * 16 + m] = acc[j][e];
+        for (int j = 0; j < NCOT; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) out[((ct0 + i) * 16 + kq * 4 + e) * Cout + j * 16 + m] = acc[i][j][e];
+    }
     if (want_bias) {  // bias gradient of this band: sum the 4 lane groups holding the same channel
 #pragma unroll
         for (int j = 0; j < NCOT; ++j) {
@@ -340,13 +362,19 @@ int npde_conv3x3_wgrad(const float* x, const float* g, int B, int H, int W, int 
         ((rows * W) % 16) || cin <= 0 || cout <= 0 || (cin % 16) || (cout % 16) || cin > 256 || cout > 128)
         return 1;
     const int nbands = B * (H / rows);
-    const int waves = nbands * 9 * (cin / 16);
+    const int nc16 = cin / 16;
+    const int nci = nc16 % 4 == 0 ? 4 : (nc16 % 2 == 0 ? 2 : 1);  // input-channel tiles per wavefront
+    const int waves = nbands * 9 * (nc16 / nci);
     const dim3 grid(unsigned((waves + 3) / 4)), block(256);
     hipStream_t s = (hipStream_t)stream;
     using WgradFn = void (*)(const float*, const float*, int, int, int, int, int, float*, float*);
-    static constexpr WgradFn table[8] = {k_conv_wgrad<1>, k_conv_wgrad<2>, k_conv_wgrad<3>, k_conv_wgrad<4>,
-                                         k_conv_wgrad<5>, k_conv_wgrad<6>, k_conv_wgrad<7>, k_conv_wgrad<8>};
-    hipLaunchKernelGGL(table[cout / 16 - 1], grid, block, 0, s, x, g, B, H, W, cin, rows, part, bpart);
+#define NPDE_WGRAD_ROW(NCI) \
+    {k_conv_wgrad<NCI, 1>, k_conv_wgrad<NCI, 2>, k_conv_wgrad<NCI, 3>, k_conv_wgrad<NCI, 4>, \
+     k_conv_wgrad<NCI, 5>, k_conv_wgrad<NCI, 6>, k_conv_wgrad<NCI, 7>, k_conv_wgrad<NCI, 8>}
+    static constexpr WgradFn table[3][8] = {NPDE_WGRAD_ROW(1), NPDE_WGRAD_ROW(2), NPDE_WGRAD_ROW(4)};
+#undef NPDE_WGRAD_ROW
+    hipLaunchKernelGGL(table[nci == 4 ? 2 : nci - 1][cout / 16 - 1], grid, block, 0, s, x, g, B, H, W, cin, rows, part,
+                       bpart);
     const int nw = 9 * cin * cout;
     hipLaunchKernelGGL(k_wgrad_reduce, dim3((nw + cout + 255) / 256), dim3(256), 0, s, part, bpart, nbands, nw, cout,
                        dw, db);
