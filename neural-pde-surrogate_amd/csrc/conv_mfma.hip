@@ -12,8 +12,8 @@
 // columns).  For every 32 input channels it stages the tile's input region -- the tile and a 1-pixel
 // periodic halo -- in LDS as bfloat16, once (64-byte pixel records, swizzled against bank conflicts), and
 // the wavefronts then take all nine taps from there: M = 16 output channels (the weights, pre-packed in
-// MFMA operand order, are the A operand), N = 16 pixels of the tile, K = the 32 staged channels of one tap
-// (their B operand, one 16-byte LDS read per lane).  Epilogue: + bias, optional tanh-GELU, optional
+// MFMA operand order and staged in LDS with the region, are the A operand), N = 16 pixels of the tile,
+// K = the 32 staged channels of one tap (their B operand, one 16-byte LDS read per lane).  Epilogue: + bias, optional tanh-GELU, optional
 // pre-activation output; or, for the data gradient, multiplication by GELU'(pre-activation of the
 // previous layer).  Weights: [tap][cin/32][cout/16][lane][8] (tap = (dy+1)*3 + (dx+1), cross-correlation,
 // circular padding; channels beyond cin or cout zero).
@@ -82,6 +82,11 @@ constexpr int kMaxBlocksPerWave = kTilePixels / 16 / kConvWaves;
 // over the banks.  Offsets in bfloat16 elements.
 __device__ __forceinline__ int rec_at(int p, int c) { return (p << 5) + ((c ^ ((p >> 1) & 3)) << 3); }
 
+// The weight fragments of a chunk go to LDS when they fit beside the region in 64 KiB (up to 64 output
+// channels: 36 KiB); wider layers read them from global memory (L2) at each tap.
+template <int NT>
+constexpr bool kWeightsInLds = NT <= 4;
+
 // One workgroup per tile of TH x TW output pixels (TH * TW a multiple of 16, at most 128; H % TH == 0,
 // W % TW == 0); NT = cout / 16 output-channel tiles.
 template <int NT>
@@ -91,6 +96,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const float* __restri
                                                            float* __restrict__ y, float* __restrict__ pre, int epi,
                                                            const float* __restrict__ gpre) {
     __shared__ __align__(16) unsigned short s_in[kRegionMax * 32];
+    extern __shared__ __align__(16) u32x4 s_w[];  // the chunk's 9 x NT weight fragments, 64 lanes each
     const int tiles_x = W / TW, tiles_y = H / TH;
     const int tile = blockIdx.x;
     const int b = tile / (tiles_x * tiles_y);
@@ -118,7 +124,16 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const float* __restri
     }
 
     for (int c = 0; c < nchunk; ++c) {
-        // stage channels 32c .. 32c + 31 of the region (channels past cin read as zero)
+        // stage the chunk's weight fragments (tap t, output tile nt at slot t * NT + nt) and channels
+        // 32c .. 32c + 31 of the region (channels past cin read as zero); both sets of loads are in flight
+        // together, and the taps then read everything from LDS
+        if constexpr (kWeightsInLds<NT>) {
+            for (int it = threadIdx.x; it < 9 * NT * 64; it += kConvThreads) {
+                const int frag = it >> 6, l = it & 63;
+                const int t = frag / NT, nt = frag - t * NT;
+                s_w[it] = wp[(size_t(t * nchunk + c) * NT + nt) * 64 + l];
+            }
+        }
         for (int it = threadIdx.x; it < nreg * 4; it += kConvThreads) {
             const int p = it >> 2, q = it & 3;
             const int ry = p / RW, rx = p - ry * RW;
@@ -141,7 +156,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const float* __restri
             for (int i = 0; i < kMaxBlocksPerWave; ++i)
                 if (wave + kConvWaves * i < nmb)
                     bv[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(s_in + rec_at(rp0[i] + toff, g)));
-            const u32x4* wt = wp + (size_t(t * nchunk + c) * NT) * 64 + lane;
+            const u32x4* wt = kWeightsInLds<NT> ? s_w + t * NT * 64 + lane : wp + (size_t(t * nchunk + c) * NT) * 64 + lane;
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 const bf16x8 av = __builtin_bit_cast(bf16x8, wt[nt * 64]);
@@ -332,10 +347,10 @@ int npde_conv3x3(const float* x, int B, int H, int W, int cin, const void* wpack
     const dim3 grid(unsigned(B * (H / TH) * (W / TW))), block(kConvThreads);
     hipStream_t s = (hipStream_t)stream;
     const u32x4* wp = (const u32x4*)wpack;
-#define NPDE_CONV_CASE(NT)                                                                                   \
-    case NT * 16:                                                                                            \
-        hipLaunchKernelGGL(k_conv_fwd<NT>, grid, block, 0, s, x, B, H, W, cin, TH, TW, wp, bias, cout_store, y, \
-                           pre, epilogue, gpre);                                                             \
+#define NPDE_CONV_CASE(NT)                                                                          \
+    case NT * 16:                                                                                   \
+        hipLaunchKernelGGL(k_conv_fwd<NT>, grid, block, kWeightsInLds<NT> ? 9 * NT * 64 * sizeof(u32x4) : 0, \
+                           s, x, B, H, W, cin, TH, TW, wp, bias, cout_store, y, pre, epilogue, gpre);   \
         break;
     switch (cout) {
         NPDE_CONV_CASE(1)
