@@ -291,6 +291,136 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ x,
     }
 }
 
+// LDS-staged weight gradient on v_mfma_f32_16x16x32_bf16, for bands of kWgradRows rows and up to 64 output
+// channels: one workgroup per (band, 32 input channels).  For each chunk of TW columns (16, or 8 when W = 8)
+// it stages, as bfloat16 and channel-major, the band's input region (kWgradRows + 2 rows x TW + 2 columns,
+// periodic) in three copies shifted by one column each -- so that a tap's 8 consecutive pixels are one
+// aligned 16-byte read whatever its column offset -- and the band's output gradient; the wavefronts then
+// accumulate dW[tap][ci][co] over the chunk's pixels, 32 at a time: M = 16 input channels (the A operand,
+// from the copy of the tap's column shift), N = 16 output channels (the B operand), K = 32 pixels.  The
+// (tap, input tile, output tile) combinations are dealt round-robin to the 4 wavefronts.  Channel strides
+// are padded to 4 dwords past a multiple of 32 so that the 16 lanes of a group spread over the banks.
+constexpr int kWgradRows = 8;
+constexpr int kXStride = 200;  // bfloat16 per input channel and copy: (8 + 2) rows x 16 columns, padded
+constexpr int kGStride = 136;  // bfloat16 per output channel: 8 rows x 16 columns, padded
+template <int NCOT>
+__global__ __launch_bounds__(256) void k_conv_wgrad_lds(const float* __restrict__ x, const float* __restrict__ g,
+                                                        int B, int H, int W, int cin, float* __restrict__ part) {
+    constexpr int Cout = NCOT * 16;
+    constexpr int kCombos = 9 * 2 * NCOT;
+    constexpr int kPerWave = (kCombos + 3) / 4;
+    __shared__ __align__(16) unsigned short xs[3 * 32 * kXStride];
+    __shared__ __align__(16) unsigned short gs[Cout * kGStride];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int m = lane & 15, gq = lane >> 4;
+    const int nchunks = (cin + 31) >> 5;
+    const int bands_per_img = H / kWgradRows;
+    const int band = blockIdx.x / nchunks, chunk = blockIdx.x - band * nchunks;
+    if (band >= B * bands_per_img) return;
+    const int b = band / bands_per_img, r0 = (band - b * bands_per_img) * kWgradRows;
+    const int TW = W % 16 == 0 ? 16 : 8;
+    const int RW = TW + 2;
+    const int ci0 = chunk << 5;
+    // this lane's 8 pixels of a K block: rows kb * (32 / TW) + prow, columns pcol .. pcol + 7
+    const int prow = (8 * gq) / TW, pcol = (8 * gq) % TW;
+    v4f acc[kPerWave];
+#pragma unroll
+    for (int i = 0; i < kPerWave; ++i) acc[i] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    for (int c0 = 0; c0 < W; c0 += TW) {
+        __syncthreads();  // every wavefront has read the previous chunk
+        for (int it = threadIdx.x; it < (kWgradRows + 2) * RW * 4; it += 256) {
+            const int q = it & 3, pix = it >> 2;
+            const int ry = pix / RW, rx = pix - ry * RW;
+            const int gy = wrap(r0 + ry - 1, H), gx = wrap(c0 + rx - 1, W);
+            const int ch = ci0 + 8 * q;
+            float4 lo{0.f, 0.f, 0.f, 0.f}, hi{0.f, 0.f, 0.f, 0.f};
+            if (ch < cin) {
+                const float4* src = reinterpret_cast<const float4*>(x + ((size_t(b) * H + gy) * W + gx) * cin + ch);
+                lo = src[0];
+                hi = src[1];
+            }
+            const short v[8] = {bf16_rne(lo.x), bf16_rne(lo.y), bf16_rne(lo.z), bf16_rne(lo.w),
+                                bf16_rne(hi.x), bf16_rne(hi.y), bf16_rne(hi.z), bf16_rne(hi.w)};
+#pragma unroll
+            for (int dxc = 0; dxc < 3; ++dxc) {
+                const int j = rx - dxc;  // the column of copy dxc that holds this region column
+                if (j < 0 || j >= TW) continue;
+                unsigned short* dst = xs + (dxc * 32 + 8 * q) * kXStride + ry * TW + j;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) dst[e * kXStride] = (unsigned short)v[e];
+            }
+        }
+        for (int it = threadIdx.x; it < kWgradRows * TW * (Cout / 8); it += 256) {
+            const int q = it % (Cout / 8), pix = it / (Cout / 8);
+            const int ry = pix / TW, cx = pix - ry * TW;
+            const float4* src =
+                reinterpret_cast<const float4*>(g + ((size_t(b) * H + r0 + ry) * W + c0 + cx) * Cout + 8 * q);
+            const float4 lo = src[0], hi = src[1];
+            const short v[8] = {bf16_rne(lo.x), bf16_rne(lo.y), bf16_rne(lo.z), bf16_rne(lo.w),
+                                bf16_rne(hi.x), bf16_rne(hi.y), bf16_rne(hi.z), bf16_rne(hi.w)};
+#pragma unroll
+            for (int e = 0; e < 8; ++e) gs[(8 * q + e) * kGStride + pix] = (unsigned short)v[e];
+        }
+        __syncthreads();
+        const int nkb = kWgradRows * TW / 32;
+        for (int kb = 0; kb < nkb; ++kb) {
+            const int row = kb * (32 / TW) + prow;
+#pragma unroll
+            for (int i = 0; i < kPerWave; ++i) {
+                const int cmb = wave + 4 * i;
+                if (cmb < kCombos) {  // wave-uniform
+                    const int t = cmb / (2 * NCOT), rest = cmb - t * (2 * NCOT);
+                    const int cit = rest / NCOT, cot = rest - cit * NCOT;
+                    const int dy = t / 3, dxc = t - dy * 3;
+                    const u32x4 a = *reinterpret_cast<const u32x4*>(
+                        xs + (dxc * 32 + 16 * cit + m) * kXStride + (row + dy) * TW + pcol);
+                    const u32x4 bv = *reinterpret_cast<const u32x4*>(gs + (16 * cot + m) * kGStride + kb * 32 + 8 * gq);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                                    __builtin_bit_cast(bf16x8, bv), acc[i], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // lane (m, gq) holds dW[t][ci0 + 16 cit + 4 gq + e][16 cot + m]
+#pragma unroll
+    for (int i = 0; i < kPerWave; ++i) {
+        const int cmb = wave + 4 * i;
+        if (cmb >= kCombos) break;
+        const int t = cmb / (2 * NCOT), rest = cmb - t * (2 * NCOT);
+        const int cit = rest / NCOT, cot = rest - cit * NCOT;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int ci = ci0 + 16 * cit + 4 * gq + e;
+            if (ci < cin) part[((size_t(band) * 9 + t) * cin + ci) * Cout + 16 * cot + m] = acc[i][e];
+        }
+    }
+}
+
+// Bias-gradient partials of the LDS path, one workgroup of 256 threads per band: thread (slice, co) sums g over
+// the band's pixels p = slice, slice + nslice, ... (nslice = 256 / cout), then the slices' sums are added in
+// slice order -- a fixed order, so the partials are deterministic.
+__global__ __launch_bounds__(256) void k_bias_partials(const float* __restrict__ g, int H, int W, int cout,
+                                                       float* __restrict__ bpart) {
+    __shared__ float sums[256];
+    const int nslice = 256 / cout;
+    const int slice = threadIdx.x / cout, co = threadIdx.x - slice * cout;
+    const int bands_per_img = H / kWgradRows;
+    const int b = blockIdx.x / bands_per_img, r0 = (blockIdx.x - b * bands_per_img) * kWgradRows;
+    const float* src = g + (size_t(b) * H + r0) * W * cout + co;
+    float acc = 0.f;
+    if (slice < nslice)
+        for (int p = slice; p < kWgradRows * W; p += nslice) acc += src[size_t(p) * cout];
+    sums[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < cout) {
+        float t = 0.f;
+        for (int k = 0; k < nslice; ++k) t += sums[k * cout + threadIdx.x];
+        bpart[size_t(blockIdx.x) * cout + threadIdx.x] = t;
+    }
+}
+
 // dW[k] = sum over bands (in band order) of part[band][k]; db likewise.  The band loop is unrolled so
 // that many loads are in flight while the additions keep their fixed order (deterministic gradients).
 __global__ void k_wgrad_reduce(const float* __restrict__ part, const float* __restrict__ bpart, int nbands,
@@ -377,11 +507,25 @@ int npde_conv3x3_wgrad(const float* x, const float* g, int B, int H, int W, int 
         ((rows * W) % 16) || cin <= 0 || cout <= 0 || (cin % 16) || (cout % 16) || cin > 256 || cout > 128)
         return 1;
     const int nbands = B * (H / rows);
+    hipStream_t s = (hipStream_t)stream;
+    const int nw = 9 * cin * cout;
+    if (rows == kWgradRows && cout <= 64 && (W % 16 == 0 || W == 8)) {  // the LDS-staged path
+        const dim3 grid(unsigned(nbands * ((cin + 31) / 32))), block(256);
+        switch (cout) {
+            case 16: hipLaunchKernelGGL(k_conv_wgrad_lds<1>, grid, block, 0, s, x, g, B, H, W, cin, part); break;
+            case 32: hipLaunchKernelGGL(k_conv_wgrad_lds<2>, grid, block, 0, s, x, g, B, H, W, cin, part); break;
+            case 48: hipLaunchKernelGGL(k_conv_wgrad_lds<3>, grid, block, 0, s, x, g, B, H, W, cin, part); break;
+            default: hipLaunchKernelGGL(k_conv_wgrad_lds<4>, grid, block, 0, s, x, g, B, H, W, cin, part); break;
+        }
+        hipLaunchKernelGGL(k_bias_partials, dim3(unsigned(nbands)), dim3(256), 0, s, g, H, W, cout, bpart);
+        hipLaunchKernelGGL(k_wgrad_reduce, dim3((nw + cout + 255) / 256), dim3(256), 0, s, part, bpart, nbands, nw,
+                           cout, dw, db);
+        return hipGetLastError() == hipSuccess ? 0 : 3;
+    }
     const int nc16 = cin / 16;
     const int nci = nc16 % 4 == 0 ? 4 : (nc16 % 2 == 0 ? 2 : 1);  // input-channel tiles per wavefront
     const int waves = nbands * 9 * (nc16 / nci);
     const dim3 grid(unsigned((waves + 3) / 4)), block(256);
-    hipStream_t s = (hipStream_t)stream;
     using WgradFn = void (*)(const float*, const float*, int, int, int, int, int, float*, float*);
 #define NPDE_WGRAD_ROW(NCI) \
     {k_conv_wgrad<NCI, 1>, k_conv_wgrad<NCI, 2>, k_conv_wgrad<NCI, 3>, k_conv_wgrad<NCI, 4>, \
@@ -390,7 +534,6 @@ int npde_conv3x3_wgrad(const float* x, const float* g, int B, int H, int W, int 
 #undef NPDE_WGRAD_ROW
     hipLaunchKernelGGL(table[nci == 4 ? 2 : nci - 1][cout / 16 - 1], grid, block, 0, s, x, g, B, H, W, cin, rows, part,
                        bpart);
-    const int nw = 9 * cin * cout;
     hipLaunchKernelGGL(k_wgrad_reduce, dim3((nw + cout + 255) / 256), dim3(256), 0, s, part, bpart, nbands, nw, cout,
                        dw, db);
     return hipGetLastError() == hipSuccess ? 0 : 3;
