@@ -503,8 +503,11 @@ npde_status timed_launch(npde_solver* s, int kind, F&& launch) {
 // One stage launch.  In seam mode the wavefronts' seam solves read the stage input's edge columns from
 // the edge arrays, except in stage 1 of the first step of a launch sequence (first_step: a host call, a
 // graph, an iteration loop), whose input the caller may have changed since a stage 2 last wrote them.
+// With a flux network, stage 1 always reads the state itself: an instance's first step of a run waits
+// at its first frame for the network, which need not run in a launch's first iteration, so that step
+// can come later in the launch, before any stage 2 has written the edge arrays of the new state.
 npde_status launch_stage_kernel(npde_solver* s, int stage, bool first_step) {
-    const int compact = (stage == 2 || !first_step) ? 1 : 0;
+    const int compact = (stage == 2 || (!first_step && s->net_frags == nullptr)) ? 1 : 0;
     return timed_launch(s, stage == 1 ? NPDE_PROF_STAGE1 : NPDE_PROF_STAGE2, [&] {
         return launch_stage(s->flux, s->lim, stage, s->has_corr, s->fast, s->tune.waves_per_block, s->layout,
                             s->row_waves, compact, s->band, lds_reserve(s), s->ddesc, s->dctl, s->dtasks,

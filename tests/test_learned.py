@@ -397,16 +397,25 @@ class _DevView:
 
 
 @pytest.mark.gpu
-def test_network_in_the_device_loop_equals_a_host_loop(npde_mod, cuda):
+@pytest.mark.parametrize("layout", ["row_groups", "seams", "strips"])
+def test_network_in_the_device_loop_equals_a_host_loop(npde_mod, cuda, layout):
     """npde_run_trajectory with the network set evaluates it at every frame landing of every instance: the
     frames and final states equal a host-driven loop that evaluates the network on every instance and then
-    advances to the next frame time, bit for bit (the instances land at different step counts)."""
+    advances to the next frame time, bit for bit (the instances land at different step counts), in each
+    task layout.  In the seam layout an instance's first step waits at the first frame for the network,
+    which runs in the launch's fourth iteration by default: that step must not read seam edge columns no
+    stage kernel has written yet (it once did, and the run went non-finite)."""
     m = _random_net(6, cuda, head_scale=0.3)
     m.out_scale = 0.1
     n, B, T, NOUT = 64, 4, 0.06, 6
     U0 = riemann_batch(n, range(20, 20 + B))
     grids = [npde_mod.make_grid(n, n, bc_x=1, bc_y=1)] * B
+    tuning = {"row_groups": {"row_groups": 1}, "seams": {"row_groups": 0, "seams": 2},
+              "strips": {"row_groups": 0, "seams": 0}}[layout]
     with npde_mod.Batch(grids, "hllc", "vanleer", 0.5) as a, npde_mod.Batch(grids, "hllc", "vanleer", 0.5) as h:
+        a.set_tuning(**tuning)
+        h.set_tuning(**tuning)
+        assert a.task_layout == layout
         for k in range(B):
             a.set_state(k, U0[k])
             h.set_state(k, U0[k])
