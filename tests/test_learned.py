@@ -529,3 +529,54 @@ def test_network_switches_off_where_the_signal_speed_runs_away(npde_mod, cuda):
     print(f"instance 100511: plain {plain} steps, learned {steps}", flush=True)
     assert steps[0.0][0] > 1.5 * plain and not steps[0.0][1]
     assert steps[2.5][1] and steps[2.5][0] < 0.7 * steps[0.0][0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(6))
+def test_random_flux_network_runs_are_layout_and_launch_invariant(npde_mod, cuda, seed):
+    """Seeded random flux-network runs: a batch of 2-5 instances of 64^2 or 128^2 (mixed, periodic; the network
+    tiles are 16 x 16) recorded over a few frames with the network in the device loop, under a random task layout
+    (row groups where every width is 64, seams, strips), band height, network launch period and graph length, give
+    the bits of the same run in 60-column strips with the network launched in every iteration: every frame,
+    final state, step count and switch-off flag.  Layouts, launch shapes and the network's launch period never
+    change a result, so any difference is a bug in one of the paths (stale seam edge columns, a missed release,
+    a frame written twice)."""
+    rng = np.random.default_rng(1000 + seed)
+    B = int(rng.integers(2, 6))
+    sizes = [int(rng.choice([64, 128])) for _ in range(B)]
+    if seed == 0:
+        sizes = [64] * B  # row groups apply
+    nout = int(rng.integers(3, 7))
+    T = float(rng.uniform(0.02, 0.05))
+    m = _random_net(10 + seed, cuda, head_scale=0.3)
+    m.out_scale = 0.1
+    from npde.families import riemann
+    ics = [riemann(int(rng.integers(0, 10000)), n)[0] for n in sizes]
+    grids = [npde_mod.make_grid(n, n, bc_x=1, bc_y=1) for n in sizes]
+    layouts = ["seams", "strips"] + (["row_groups"] if all(n == 64 for n in sizes) else [])
+    layout = layouts[seed % len(layouts)]
+    tuning = {"row_groups": {"row_groups": 1, "seams": 0}, "seams": {"row_groups": 0, "seams": 2},
+              "strips": {"row_groups": 0, "seams": 0}}[layout]
+    tuning.update(net_period=int(rng.integers(1, 9)), steps_per_graph=int(rng.choice([1, 3, 8, 32])),
+                  rows_per_task=int(rng.choice([0, 4, 16])))
+    runs = {}
+    for name, tu in (("reference", {"row_groups": 0, "seams": 0, "net_period": 1, "steps_per_graph": 32,
+                                    "rows_per_task": 0}), ("random", tuning)):
+        with npde_mod.Batch(grids, "hllc", "vanleer", 0.5) as b:
+            b.set_tuning(**tu)
+            if name == "random":
+                assert b.task_layout == layout, (b.task_layout, layout)
+            b.set_flux_network(m)
+            for k, U in enumerate(ics):
+                b.set_state(k, U)
+            b.run_trajectory(T, nout, sizes=tuple(sorted(set(sizes))[:1]))
+            m0 = min(sizes)
+            runs[name] = ([b.trajectory(k, m0) for k in range(B)], [b.get_state(k) for k in range(B)],
+                          b.times()[1].copy(), b.flux_network_switched_off().copy())
+    (fa, sa, na, oa), (fb, sb, nb, ob) = runs["reference"], runs["random"]
+    assert np.array_equal(na, nb), (seed, tuning, na, nb)
+    assert np.array_equal(oa, ob), seed
+    for k in range(B):
+        assert np.all(np.isfinite(sa[k])), (seed, k)
+        assert np.array_equal(fa[k], fb[k]), (seed, tuning, k)
+        assert np.array_equal(sa[k], sb[k]), (seed, tuning, k)
