@@ -190,8 +190,8 @@ bool valid_grid(const npde_grid_desc& g, std::string& why) {
     return true;
 }
 
-// Band height: the tuned value, or (rows_per_task <= 0) the tallest band in [4, 64] that still gives
-// about 2048 tasks over the listed instances, so that small grids are not bound by one wavefront
+// Band height: the tuned value, or (rows_per_task <= 0) the tallest band in [4, 64] ([2, 64] for flux-
+// corrected runs) that still gives about 2048 tasks over the listed instances, so that small grids are not bound by one wavefront
 // marching many rows, while bands stay tall enough not to recompute their 2 + 2 edge rows too often
 // (measured best or within 5% of it from a single 512^2 instance to 64 x 512^2).  When even 4-row
 // bands leave fewer than 512 tasks, the GPU is far from full and 2-row bands are faster (a single
@@ -202,13 +202,17 @@ int band_rows(const npde_solver* s, int strip_cols) {
     long long rows_total = 0;
     for (int k = 0; k < s->n; ++k)
         if (s->listed[k]) rows_total += (long long)((s->grids[k].nx + strip_cols - 1) / strip_cols) * s->grids[k].ny;
+    // flux-corrected stages (learned increments) do more work on every row of a band's serial march, so
+    // they take bands down to 2 rows while fewer than 2048 tasks result (64 x 64^2 learned rollout: 33.7
+    // against 36.7 ms with 4-row bands; the plain rollout of the same batch gains nothing)
+    const int rmin = s->has_corr ? 2 : 4;
     int r = 64;
-    while (r > 4 && rows_total / r < 2048) r /= 2;
+    while (r > rmin && rows_total / r < 2048) r /= 2;
     if (r == 4 && rows_total / 4 < 512) return 2;
     // bands shorter than 16 rows spend too much of their march on the 2 + 2 edge rows: double them while
     // 1024 tasks remain (1 x 1024^2: 82.4 -> 79.7 us per step; 2 x 512^2: 53.5 -> 50.9; 4 x 512^2 and
     // 16 x 256^2: ~1% faster); from 16 rows on, doubling measured 3-4% slower (2 x 1024^2, 8 x 512^2)
-    if (r < 16 && rows_total / (2 * r) >= 1024) r *= 2;
+    if (!s->has_corr && r < 16 && rows_total / (2 * r) >= 1024) r *= 2;
     return r;
 }
 
@@ -1182,9 +1186,11 @@ npde_status npde_set_flux_correction(npde_solver* s, int inst, const double* cor
         D.corr_x = D.corr_y = nullptr;
         return fail(s, NPDE_ERR_INVALID, "flux correction needs nx*ny to be a multiple of 64");
     }
+    const bool changed = s->has_corr != any;
     s->has_corr = any;
-    HIPCHK(s, hipMemcpy(s->ddesc, s->hdesc.data(), sizeof(InstDesc) * s->n, hipMemcpyHostToDevice));
     drop_graph(s);
+    if (changed) return upload_tasks(s);  // the automatic band height depends on it; uploads the descriptors
+    HIPCHK(s, hipMemcpy(s->ddesc, s->hdesc.data(), sizeof(InstDesc) * s->n, hipMemcpyHostToDevice));
     return NPDE_OK;
 }
 
@@ -1208,8 +1214,10 @@ npde_status npde_set_flux_network(npde_solver* s, const void* weights, const flo
         s->net_bias = nullptr;
         bool any = false;
         for (const auto& d : s->hdesc) any = any || d.corr_x != nullptr;
+        const bool changed = s->has_corr != any;
         s->has_corr = any;
         drop_graph(s);
+        if (changed) return upload_tasks(s);
         HIPCHK(s, hipMemcpy(s->ddesc, s->hdesc.data(), sizeof(InstDesc) * s->n, hipMemcpyHostToDevice));
         return NPDE_OK;
     }
@@ -1242,8 +1250,10 @@ npde_status npde_set_flux_network(npde_solver* s, const void* weights, const flo
     s->net_frags = weights;
     s->net_bias = biases;
     s->net_scale = float(out_scale);
+    const bool changed = !s->has_corr;
     s->has_corr = true;
     drop_graph(s);
+    if (changed) return upload_tasks(s);
     HIPCHK(s, hipMemcpy(s->ddesc, s->hdesc.data(), sizeof(InstDesc) * s->n, hipMemcpyHostToDevice));
     return NPDE_OK;
 }
