@@ -88,11 +88,11 @@ __device__ __forceinline__ void cell_features(double d, double m, double n, doub
 //          beyond the ninth are zero);
 //   HEAD:  8 output channels, out_scale * tanh written as binary64 factors into the instance's
 //          correction planes (channels 0-3: face left of the cell, 4-7: face below it).
-template <int KB, int NT, bool FIRST, bool HEAD>
+template <int KB, int NT, bool FIRST, bool HEAD, int Win>
 __device__ __forceinline__ void net_layer(const unsigned short* __restrict__ in, unsigned short* __restrict__ out,
-                                          int Win, const u32x4* __restrict__ frags, const float* __restrict__ bias,
+                                          const u32x4* __restrict__ frags, const float* __restrict__ bias,
                                           int wave, int lane, const InstDesc& D, int gx0, int gy0, float out_scale) {
-    const int Wout = Win - 2;
+    constexpr int Wout = Win - 2;  // compile-time region widths: the cell -> (row, column) divisions are constant
     const int ncell = Wout * Wout;
     const int nblk = (ncell + 15) >> 4;
     const int m = lane & 15, g = lane >> 4;
@@ -187,18 +187,18 @@ __device__ __forceinline__ void net_tile(const InstDesc& D, const NetTask& tk, u
     for (int o = 32; o > 0; o >>= 1) dmax = fmaxf(dmax, __shfl_xor(dmax, o, kWave));
     if (peak && lane == 0) atomicMax(peak + tk.inst, __float_as_uint(dmax));
     __syncthreads();
-    net_layer<3, 2, true, false>(bufB, bufA, 26, frags + kFragFirst * kWave, bias, wave, lane, D, 0, 0, 0.f);
+    net_layer<3, 2, true, false, 26>(bufB, bufA, frags + kFragFirst * kWave, bias, wave, lane, D, 0, 0, 0.f);
     __syncthreads();
-    net_layer<9, 2, false, false>(bufA, bufB, 24, frags + kFragHidden * kWave, bias + 32, wave, lane, D, 0, 0, 0.f);
+    net_layer<9, 2, false, false, 24>(bufA, bufB, frags + kFragHidden * kWave, bias + 32, wave, lane, D, 0, 0, 0.f);
     __syncthreads();
-    net_layer<9, 2, false, false>(bufB, bufA, 22, frags + (kFragHidden + 18) * kWave, bias + 64, wave, lane, D, 0,
-                                  0, 0.f);
+    net_layer<9, 2, false, false, 22>(bufB, bufA, frags + (kFragHidden + 18) * kWave, bias + 64, wave, lane, D, 0, 0,
+                                      0.f);
     __syncthreads();
-    net_layer<9, 2, false, false>(bufA, bufB, 20, frags + (kFragHidden + 36) * kWave, bias + 96, wave, lane, D, 0,
-                                  0, 0.f);
+    net_layer<9, 2, false, false, 20>(bufA, bufB, frags + (kFragHidden + 36) * kWave, bias + 96, wave, lane, D, 0, 0,
+                                      0.f);
     __syncthreads();
-    net_layer<9, 1, false, true>(bufB, nullptr, 18, frags + kFragHead * kWave, bias + 128, wave, lane, D, tk.x0,
-                                 tk.y0, out_scale);
+    net_layer<9, 1, false, true, 18>(bufB, nullptr, frags + kFragHead * kWave, bias + 128, wave, lane, D, tk.x0, tk.y0,
+                                     out_scale);
 }
 
 // Workgroups stride over the tile list.  counters == null: every instance's tiles.  Otherwise (the device
