@@ -351,7 +351,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the batch would shard across GPUs "
-              "by instances with no communication, which this build does not run.", file=sys.stderr)
+              "by instances with no communication; the multi-GPU launch is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -12,7 +12,7 @@ namespace npde_io {
 #ifndef NPDE_WITH_HDF5
 
 bool available() { return false; }
-static void no_hdf5() { throw std::runtime_error("this build has no HDF5 support"); }
+static void no_hdf5() { throw std::runtime_error("libnpde_io was compiled without HDF5 support"); }
 void write_snapshot(const std::string&, const npde_grid_desc&, const std::vector<double>&, double, const RunInfo&) {
     no_hdf5();
 }
