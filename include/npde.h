@@ -196,12 +196,12 @@ npde_status npde_builtin_problem(const char* test, int n, npde_grid_desc* grid, 
 /* ---- learned-correction network kernels (no counterpart in the reference: its networks are JAX;
  *      closest analog the periodic 3x3 convolution of surrogate/models.py:16-29) ----
  * Device pointers; activations NHWC float32, values rounded to bfloat16 at the MFMA inputs, FP32
- * accumulation.  npde_conv3x3: input channels a multiple of 8 (x 16-byte aligned), output channels 16 to
- * 128 or 192 in steps of 16, W a multiple of 8; wpack: bfloat16 weights in the MFMA A-operand order
- * [tap][cin/32][cout/16][64][8] (zero past cin and cout).  epilogue: 0 linear (+bias), 1 +bias then
- * tanh-GELU (pre-activation to `pre` if non-NULL), 2 multiply by GELU'(gpre) (data gradient).
- * npde_conv3x3_wgrad: input channels a multiple of 16 up to 256, output channels up to 128, rows * W a
- * multiple of 16.  Return 0 on success. */
+ * accumulation.  npde_conv3x3: input channels a multiple of 8 (x and wpack 16-byte aligned), output
+ * channels 16 to 128 or 192 in steps of 16, W a multiple of 8; wpack: bfloat16 weights in the MFMA
+ * A-operand order [tap][cin/32][cout/16][64][8] (zero past cin and cout).  epilogue: 0 linear (+bias),
+ * 1 +bias then tanh-GELU (pre-activation to `pre` if non-NULL), 2 multiply by GELU'(gpre) (data gradient).
+ * npde_conv3x3_wgrad: input channels a multiple of 16 up to 256, output channels up to 128, W a multiple
+ * of 4, rows * W a multiple of 16.  Return 0 on success, 1 on shapes or pointers outside these limits. */
 int npde_conv3x3(const float* x, int B, int H, int W, int cin, const void* wpack, const float* bias, int cout,
                  int cout_store, float* y, float* pre, int epilogue, const float* gpre, void* stream);
 /* Weight/bias gradient of a 3x3 periodic convolution: dw[tap][ci][co], db[co], summed over bands of

@@ -171,6 +171,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const float* __restri
 
     // lane (m, g) holds output channels 16 nt + 4g .. + 3 of tile pixel 16 mb + m
     const int cout = NT * 16;
+    const bool vec_store = (cout_store & 3) == 0 && (reinterpret_cast<uintptr_t>(y) & 15) == 0;
 #pragma unroll
     for (int i = 0; i < kMaxBlocksPerWave; ++i) {
         const int mb = wave + kConvWaves * i;
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const float* __restri
                 else if (epi == kGeluGrad) z *= gelu_grad(gpre[gp * cout + co + e]);
                 v[e] = z;
             }
-            if ((cout_store & 3) == 0 && co + 4 <= cout_store) {
+            if (vec_store && co + 4 <= cout_store) {
                 *reinterpret_cast<float4*>(y + gp * cout_store + co) = float4{v[0], v[1], v[2], v[3]};
             } else {
 #pragma unroll
@@ -203,7 +204,8 @@ __global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const float* __restri
 
 // Weight gradient partials: one wavefront per (band of `rows` image rows, tap, group of NCI x 16 input
 // channels; NCI = 4 when cin is a multiple of 64, else 2 or 1).  dW[t][ci][co] = sum over the band's pixels p of x[p + off(t)][ci] * g[p][co], 16 pixels
-// at a time in row-major order within the band (rows * W a multiple of 16); the (centre tap, first group)
+// at a time in row-major order within the band (rows * W a multiple of 16, W a multiple of 4, so that each
+// lane's 4 consecutive pixels of a K block lie in one image row); the (centre tap, first group)
 // wavefront also sums g over the band (bias gradient).  M = 16 input channels, N = 16 output channels,
 // K = 16 pixels per v_mfma_f32_16x16x16_bf16; the gradient values of a K block are loaded once for all the
 // group's channel tiles.
@@ -235,24 +237,22 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ x,
         for (int i = 0; i < NCI; ++i) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
         bsum[j] = 0.f;
     }
-    // the K blocks walk the band 16 pixels at a time: (rb, xb) is the block's first pixel; this lane's 4 pixels
-    // (k index 4 kq + i) sit lr rows and lx + i columns past it (W = 8: a block spans two rows)
-    const int lr = (4 * kq) / W, lx = (4 * kq) % W;
-    int rb = r0, xb = 0;
+    // the K blocks walk the band 16 pixels at a time in row-major order; (r, xl) is the first of this lane's
+    // 4 pixels of the current block (k index 4 kq + i: row r, columns xl + i), advanced by 16 pixels per block
+    int r = r0 + (4 * kq) / W, xl = (4 * kq) % W;
     for (int blk = rows * W / 16; blk > 0; --blk) {
         size_t ps[4], pd[4];
-        const int r = rb + lr;
         const size_t rd = (size_t(b) * H + r) * W, rs = (size_t(b) * H + wrap(r + dy, H)) * W;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int xd = xb + lx + i;
+            const int xd = xl + i;
             pd[i] = rd + xd;
             ps[i] = rs + wrap(xd + dx, W);
         }
-        xb += 16;
-        while (xb >= W) {
-            xb -= W;
-            ++rb;
+        xl += 16;
+        while (xl >= W) {
+            xl -= W;
+            ++r;
         }
         v4s av[NCI];
 #pragma unroll
@@ -460,13 +460,14 @@ using namespace npde_nn;
 
 extern "C" {
 
-// Forward / data-gradient convolution (see the file header).  cin a multiple of 8 (x 16-byte aligned);
-// cout 16 .. 128 or 192 in steps of 16; W a multiple of 8; H a multiple of the tile's rows (8 or 16, or
-// H itself when smaller).
+// Forward / data-gradient convolution (see the file header).  cin a multiple of 8 (x and wpack 16-byte
+// aligned); cout 16 .. 128 or 192 in steps of 16; W a multiple of 8.  A tile is 16 columns wide when 16
+// divides W, else 8, and takes the most rows (up to 128 pixels) that divide H with a multiple of 16 pixels.
 int npde_conv3x3(const float* x, int B, int H, int W, int cin, const void* wpack, const float* bias, int cout,
                  int cout_store, float* y, float* pre, int epilogue, const float* gpre, void* stream) {
     if (!x || !wpack || !y || B <= 0 || H <= 0 || W <= 0 || (W % 8) || (cin % 8) || cin <= 0 ||
-        (reinterpret_cast<uintptr_t>(x) & 15) || cout_store <= 0 || cout_store > cout ||
+        ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(wpack)) & 15) || cout_store <= 0 ||
+        cout_store > cout ||
         (epilogue == kGeluGrad && !gpre))
         return 1;
     // tile: 16 columns when the width allows, else 8; as many rows as divide H up to 128 pixels
@@ -498,18 +499,22 @@ int npde_conv3x3(const float* x, int B, int H, int W, int cin, const void* wpack
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 
-// Weight and bias gradients.  cin a multiple of 16 up to 256, cout a multiple of 16 up to 128; rows divides
-// H and rows * W is a multiple of 16.  part must hold (B*H/rows) * 9 * cin * cout floats and bpart
-// (B*H/rows) * cout; dw is written as [tap][ci][co], db as [co].
+// Weight and bias gradients.  cin a multiple of 16 up to 256, cout a multiple of 16 up to 128; W a multiple
+// of 4; rows divides H and rows * W is a multiple of 16.  part must hold (B*H/rows) * 9 * cin * cout floats
+// and bpart (B*H/rows) * cout; dw is written as [tap][ci][co], db as [co].  The LDS-staged kernel takes
+// bands of 8 rows, up to 64 output channels, widths that are multiples of 8 and 16-byte aligned x and g;
+// other shapes take the global-load kernel.
 int npde_conv3x3_wgrad(const float* x, const float* g, int B, int H, int W, int cin, int cout, int rows,
                        float* part, float* bpart, float* dw, float* db, void* stream) {
-    if (!x || !g || !part || !bpart || !dw || !db || rows <= 0 || B <= 0 || H <= 0 || W <= 0 || (H % rows) ||
-        ((rows * W) % 16) || cin <= 0 || cout <= 0 || (cin % 16) || (cout % 16) || cin > 256 || cout > 128)
+    if (!x || !g || !part || !bpart || !dw || !db || rows <= 0 || B <= 0 || H <= 0 || W <= 0 || (W % 4) ||
+        (H % rows) || ((rows * W) % 16) || cin <= 0 || cout <= 0 || (cin % 16) || (cout % 16) || cin > 256 ||
+        cout > 128)
         return 1;
     const int nbands = B * (H / rows);
     hipStream_t s = (hipStream_t)stream;
     const int nw = 9 * cin * cout;
-    if (rows == kWgradRows && cout <= 64 && (W % 16 == 0 || W == 8)) {  // the LDS-staged path
+    const bool aligned = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(g)) & 15) == 0;
+    if (rows == kWgradRows && cout <= 64 && W % 8 == 0 && aligned) {  // the LDS-staged path
         const dim3 grid(unsigned(nbands * ((cin + 31) / 32))), block(256);
         switch (cout) {
             case 16: hipLaunchKernelGGL(k_conv_wgrad_lds<1>, grid, block, 0, s, x, g, B, H, W, cin, part); break;
