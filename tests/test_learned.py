@@ -212,14 +212,16 @@ def test_network_forward_and_gradients_match_oracle(cuda):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("B,H,W,cin,cout,fwd", [(2, 16, 24, 32, 32, True), (1, 8, 40, 16, 48, True),
-                                                 (2, 8, 24, 32, 96, True), (1, 16, 12, 16, 16, False)])
-def test_layer_kernels_match_float64_at_widths_not_a_multiple_of_16(cuda, B, H, W, cin, cout, fwd):
+@pytest.mark.parametrize("B,H,W,cin,cout,fwd,wgrad", [(2, 16, 24, 32, 32, True, True), (1, 8, 40, 16, 48, True, True),
+                                                       (2, 8, 24, 32, 96, True, True), (1, 16, 12, 16, 16, False, True),
+                                                       (1, 12, 16, 16, 16, True, False), (1, 20, 8, 16, 32, True, False)])
+def test_layer_kernels_match_float64_at_widths_not_a_multiple_of_16(cuda, B, H, W, cin, cout, fwd, wgrad):
     """npde_conv3x3 and npde_conv3x3_wgrad at image widths that are not multiples of 16 (8-column tiles; the
     weight gradient's K blocks of 16 pixels straddle image rows at varying offsets), through both weight-gradient
-    kernels (LDS-staged up to 64 output channels, global loads beyond, and widths that are multiples of 4 only):
-    against float64 periodic convolutions of the same bfloat16-rounded operands.  Products of two bfloat16 values
-    are exact in float32, so only the float32 accumulation differs (relative 1e-5)."""
+    kernels (LDS-staged up to 64 output channels, global loads beyond, and widths that are multiples of 4 only),
+    and the forward at heights that are not multiples of 8 (tiles of 6 x 16 and 10 x 8 pixels): against float64
+    periodic convolutions of the same bfloat16-rounded operands.  Products of two bfloat16 values are exact in
+    float32, so only the float32 accumulation differs (relative 1e-5)."""
     import torch.nn.functional as Fn
     from npde.learned import conv as K
     g_ = torch.Generator().manual_seed(B * 1000 + W * 10 + cout)
@@ -234,6 +236,8 @@ def test_layer_kernels_match_float64_at_widths_not_a_multiple_of_16(cuda, B, H, 
         y, _ = K.conv3x3(x.to(cuda), K.pack_weights(w.to(cuda)), bias.to(cuda), cout, cout, K.EPI_LINEAR)
         y_ref = (Fn.conv2d(xpad, bf(w)) + bias.double().view(1, -1, 1, 1)).permute(0, 2, 3, 1)
         assert rel(y, y_ref) < 1e-5
+    if not wgrad:
+        return
     wv = torch.zeros(cout, cin, 3, 3, dtype=torch.float64, requires_grad=True)
     (Fn.conv2d(xpad, wv) * bf(g).permute(0, 3, 1, 2)).sum().backward()
     dw, db = K.conv3x3_wgrad(x.to(cuda), g.to(cuda))
