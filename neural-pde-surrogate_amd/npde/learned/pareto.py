@@ -45,6 +45,14 @@ def _metrics(pred: np.ndarray, truth: np.ndarray, std: np.ndarray) -> dict:
     return {"rmse_last": float(np.median(last)), "l1_rho": float(np.mean(l1)), "drift_max": drift, "bad_cells": bad}
 
 
+def pareto_flags(rows) -> list:
+    """Per row, True when no other row is at least as cheap and at least as accurate (last-frame RMSE) and
+    strictly better in one of the two: the non-dominated points of the accuracy-versus-cost plane that
+    experiments/pareto.py:39-56 plots."""
+    return [not any(o is not r and o["cost"] <= r["cost"] and o["rmse_last"] <= r["rmse_last"]
+                    and (o["cost"] < r["cost"] or o["rmse_last"] < r["rmse_last"]) for o in rows) for r in rows]
+
+
 def run(instances=64, ladder=(32, 64, 128, 256), learned=True, log=print) -> dict:
     idx = list(range(100000, 100000 + instances))
     t0 = time.perf_counter()
@@ -74,9 +82,8 @@ def run(instances=64, ladder=(32, 64, 128, 256), learned=True, log=print) -> dic
         log(f"  {row['method']:>14}: {row['cost'] * 1e3:8.3f} ms per trajectory  rmse_last {row['rmse_last']:.4f}  "
             f"L1(rho) {row['l1_rho']:.5f}  fallbacks {len(fell)}")
     # the Pareto front: methods no other method beats in both cost and last-frame accuracy
-    for r in rows:
-        r["pareto"] = not any(o is not r and o["cost"] <= r["cost"] and o["rmse_last"] <= r["rmse_last"]
-                              and (o["cost"] < r["cost"] or o["rmse_last"] < r["rmse_last"]) for o in rows)
+    for r, front in zip(rows, pareto_flags(rows)):
+        r["pareto"] = front
     return {"instances": instances, "t_end": P.T_END, "frames": P.NOUT, "truth": f"{P.FINE}^2 block-averaged",
             "std": std.tolist(), "rows": rows}
 
