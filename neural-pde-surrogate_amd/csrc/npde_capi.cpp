@@ -125,10 +125,6 @@ struct npde_solver {
     float* scratch = nullptr;  // that GPU's copy of the static staging array (npde_block_average_noalloc)
 };
 
-// Staging buffer of npde_block_average_noalloc: a file-scope device array, so that the call allocates
-// nothing.  Every GPU the library runs on holds its own copy, at its own address, until the process
-// ends; the host-side name `scratch` is not a device address, so each handle keeps its device's address
-// (looked up once per device by bind_scratch) and passes that to kernels and copies.
 // Flux-network runs switch the network off for an instance whose CFL signal-speed sum grows beyond this
 // multiple of its first frame's (npde_set_flux_network_limit).  Of 512 held-out random-Riemann instances
 // at 64^2, the corrected flows of the eight slowest reach 1.4-2.2x where they stay healthy; the two that
@@ -138,6 +134,10 @@ struct npde_solver {
 // instead of 0.27 s.
 constexpr double kDefaultNetRateLimit = 2.5;
 
+// Staging buffer of npde_block_average_noalloc: a file-scope device array, so that the call allocates
+// nothing.  Every GPU the library runs on holds its own copy, at its own address, until the process
+// ends; the host-side name `scratch` is not a device address, so each handle keeps its device's address
+// (looked up once per device by bind_scratch) and passes that to kernels and copies.
 constexpr size_t kNoAllocFloats = size_t(4) * 512 * 512;  // 4 MiB: four 512 x 512 float32 planes
 static __device__ __align__(256) float scratch[kNoAllocFloats];
 
