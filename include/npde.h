@@ -168,7 +168,9 @@ npde_status npde_set_flux_correction(npde_solver* s, int instance, const double*
  * npde_advance_to keeps the factors of the last evaluation.  weights: device pointer to the 69 packed bfloat16 MFMA fragments of
  * the five layers (64 lanes x 8 values each: first layer 3 k-blocks x 2 n-tiles, three hidden layers 9 x 2,
  * head 9 x 1; npde.learned.model.FluxCorrector.pack_fused writes them); biases: device pointer to
- * 32 + 32 + 32 + 32 + 8 float32.  Both stay the caller's and must stay alive while the network is set.
+ * 32 + 32 + 32 + 32 + 8 float32.  Both stay the caller's and must stay alive while the network is set;
+ * the library's stream does not wait for the caller's streams, so their contents must be complete on the
+ * device before this call (e.g. the caller synchronises the stream that wrote them).
  * Every instance must be periodic with nx and ny multiples of 16.  weights = NULL switches it off. */
 npde_status npde_set_flux_network(npde_solver* s, const void* weights, const float* biases, double out_scale);
 /* In npde_run_trajectory with the network set: switch the network off for an instance (its factors are

@@ -307,7 +307,14 @@ class Batch:
             check(self._h, self._lib.npde_set_flux_network(self._h, None, None, 0.0))
             self._net = None
             return
+        import torch
         frags, biases, scale = model.pack_fused()
+        if frags.device.type != "cuda" or biases.device != frags.device:
+            raise ValueError("set_flux_network needs the model on the GPU (the kernels read its packed weights "
+                             "from device memory)")
+        # packed by torch on its own stream, which the library's stream does not wait for: complete them
+        # before the library can read them
+        torch.cuda.current_stream(frags.device).synchronize()
         self._net = (frags, biases)  # device buffers the library reads until switched off
         check(self._h, self._lib.npde_set_flux_network(self._h, C.c_void_p(frags.data_ptr()),
                                                        C.c_void_p(biases.data_ptr()), scale))
