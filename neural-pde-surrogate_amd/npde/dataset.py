@@ -24,6 +24,7 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import os
 import time
 from pathlib import Path
 
@@ -136,8 +137,12 @@ def generate(items, sizes, riemann_n=512, batch=256, log=print, source="benchmar
     return out, time.perf_counter() - t0
 
 
+IO_LIB_PATH = PKG_ROOT / "lib" / "libnpde_io.so"
+
+
 def _io():
-    lib = C.CDLL(str(PKG_ROOT / "lib" / "libnpde_io.so"))
+    # NPDE_IO_LIB: another build of the writer (an instrumented one, for example)
+    lib = C.CDLL(os.environ.get("NPDE_IO_LIB", str(IO_LIB_PATH)))
     lib.npde_io_available.restype = C.c_int
     lib.npde_io_last_error.restype = C.c_char_p
     lib.npde_io_write_packed.restype = C.c_int
