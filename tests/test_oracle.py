@@ -5,7 +5,8 @@
 2. The reference build itself (oracle/_ref), where present, on inputs the fixtures do not hold.
 3. What the mathematics fixes: consistency F(U, U) = F(U), rotation symmetry of the y faces, exact
    x/y transposition of Sod, free-stream preservation, conservation, the exact Sod plateau, the
-   published convergence table and step counts, and closed forms of the limiters.
+   published convergence table (32^2 to 256^2, L1 errors and the van Leer L-infinity order), the
+   published Kelvin-Helmholtz growth table and step counts, and closed forms of the limiters.
 """
 import numpy as np
 import pytest
@@ -274,17 +275,41 @@ def test_free_stream_preserved(oracle):
     assert np.abs(U - U0[:, None, None]).max() < 1e-13
 
 
-@pytest.mark.parametrize("n,lim,l1", [(32, "vanleer", 1.088e-2), (64, "vanleer", 3.203e-3),
-                                      (32, "none", 9.024e-2), (64, "none", 5.862e-2)])
-def test_published_convergence_table(oracle, n, lim, l1):
-    """experiments/results.md:75-80 (L1 density error of sine2d at t = 1, printed to 4 digits)."""
-    g, t_end, U = oracle.setup("sine2d", n)
-    out = oracle.run(g, "hllc", lim, 0.5, U, t_end, nout=1)
-    x = (np.arange(n) + 0.5) / n
-    X, Y = np.meshgrid(x, x)
-    exact = 1.0 + 0.2 * np.sin(2 * np.pi * (X + Y - 2 * out["t"]))
-    err = np.abs(U[0] - exact).sum() * (1 / n) * (1 / n)
-    assert float(f"{err:.3e}") == l1
+# experiments/results.md:66-74: L1 density error of the diagonal sine wave at t = 1 (printed to 4 digits)
+# and the L-infinity order of van Leer between successive sizes (printed to 2 decimals)
+SINE2D_L1 = {("vanleer", 32): 1.088e-2, ("vanleer", 64): 3.203e-3, ("vanleer", 128): 8.531e-4,
+             ("vanleer", 256): 2.130e-4, ("none", 32): 9.024e-2, ("none", 64): 5.862e-2,
+             ("none", 128): 3.379e-2, ("none", 256): 1.820e-2}
+
+
+@pytest.fixture(scope="module")
+def sine2d_errors(oracle):
+    """(L1, L-infinity) density errors of every run of the published table."""
+    errs = {}
+    for (lim, n) in SINE2D_L1:
+        g, t_end, U = oracle.setup("sine2d", n)
+        out = oracle.run(g, "hllc", lim, 0.5, U, t_end, nout=1)
+        x = (np.arange(n) + 0.5) / n
+        X, Y = np.meshgrid(x, x)
+        exact = 1.0 + 0.2 * np.sin(2 * np.pi * (X + Y - 2 * out["t"]))
+        e = np.abs(U[0] - exact)
+        errs[(lim, n)] = (e.sum() * (1 / n) * (1 / n), e.max())
+    return errs
+
+
+@pytest.mark.parametrize("lim,n", list(SINE2D_L1))
+def test_published_convergence_table(sine2d_errors, lim, n):
+    """The diagonal-advection table of experiments/results.md:66-74, HLLC, SSP-RK2, CFL 0.5, 32^2 to 256^2."""
+    assert float(f"{sine2d_errors[(lim, n)][0]:.3e}") == SINE2D_L1[(lim, n)]
+
+
+def test_published_linf_order_of_van_leer(sine2d_errors):
+    """results.md:70-74: the L-infinity order of van Leer stays at 1.41 from 64^2 to 256^2 (clipping at the
+    smooth extrema), while its L1 order approaches 2."""
+    for n in (64, 128, 256):
+        order = np.log2(sine2d_errors[("vanleer", n // 2)][1] / sine2d_errors[("vanleer", n)][1])
+        assert round(float(order), 2) == 1.41, n
+    assert round(float(np.log2(sine2d_errors[("vanleer", 128)][0] / sine2d_errors[("vanleer", 256)][0])), 2) == 2.00
 
 
 def kh_mode_amplitude(U, n):
