@@ -275,7 +275,7 @@ def test_free_stream_preserved(oracle):
     assert np.abs(U - U0[:, None, None]).max() < 1e-13
 
 
-# experiments/results.md:66-74: L1 density error of the diagonal sine wave at t = 1 (printed to 4 digits)
+# experiments/results.md:71-80: L1 density error of the diagonal sine wave at t = 1 (printed to 4 digits)
 # and the L-infinity order of van Leer between successive sizes (printed to 2 decimals)
 SINE2D_L1 = {("vanleer", 32): 1.088e-2, ("vanleer", 64): 3.203e-3, ("vanleer", 128): 8.531e-4,
              ("vanleer", 256): 2.130e-4, ("none", 32): 9.024e-2, ("none", 64): 5.862e-2,
@@ -299,12 +299,12 @@ def sine2d_errors(oracle):
 
 @pytest.mark.parametrize("lim,n", list(SINE2D_L1))
 def test_published_convergence_table(sine2d_errors, lim, n):
-    """The diagonal-advection table of experiments/results.md:66-74, HLLC, SSP-RK2, CFL 0.5, 32^2 to 256^2."""
+    """The diagonal-advection table of experiments/results.md:71-80, HLLC, SSP-RK2, CFL 0.5, 32^2 to 256^2."""
     assert float(f"{sine2d_errors[(lim, n)][0]:.3e}") == SINE2D_L1[(lim, n)]
 
 
 def test_published_linf_order_of_van_leer(sine2d_errors):
-    """results.md:70-74: the L-infinity order of van Leer stays at 1.41 from 64^2 to 256^2 (clipping at the
+    """results.md:75-80: the L-infinity order of van Leer stays at 1.41 from 64^2 to 256^2 (clipping at the
     smooth extrema), while its L1 order approaches 2."""
     for n in (64, 128, 256):
         order = np.log2(sine2d_errors[("vanleer", n // 2)][1] / sine2d_errors[("vanleer", n)][1])
