@@ -95,10 +95,13 @@ def test_trajectory_block_averages_match_reference_cli(oracle, golden):
 
 
 def test_published_step_count_lax_liu_256(oracle):
-    """experiments/results.md:92-94: 682 steps at 256^2, t = 0.3, nout = 10."""
+    """experiments/results.md:92-94: 682 steps at 256^2, t = 0.3, nout = 10; and results.md:86-88 (README.md:64):
+    the final density is symmetric about x = y to ~3e-14 (to one digit: not exact, the x-then-y accumulation
+    of the update breaks the mirror symmetry at rounding level)."""
     g, t_end, U = oracle.setup("lax_liu3", 256)
     out = oracle.run(g, "hllc", "vanleer", 0.5, U, t_end, nout=10)
     assert out["steps"] == 682
+    assert f"{np.abs(U[0] - U[0].T).max():.0e}" == "3e-14"
 
 
 # ---------------------------------------------------------------- 2. the reference build itself
