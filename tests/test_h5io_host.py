@@ -4,8 +4,10 @@ snapshot of the reference's own I/O test and reads files back, and h5dump checks
 
 - The reference's test_io2d.cpp writes a snapshot with known values on a non-square, non-unit domain and
   tests/check_snapshot.py checks every value, shape and attribute: the same snapshot and checks here.
-- A snapshot with the attributes of the reference CLI's 32^2 sine2d snapshot has its storage and attribute
-  layout (h5dump -p -A, tests/golden/h5_layout.json) character for character.
+- A snapshot with the attributes of the reference CLI's 32^2 sine2d snapshot, and a trajectory file of the
+  oracle's Lax-Liu 32^2 frames (equal to the reference CLI's), have the storage and attribute layouts of the
+  reference CLI's files (h5dump -p -A, tests/golden/h5_layout.json; compressed sizes included) character
+  for character.
 - Reading a snapshot back as an initial condition (io_hdf5.hpp:171-175) gives the grid and the conserved
   state that to_conserved (state2d.hpp:16-19, the oracle's restatement) makes of its primitive datasets,
   bit for bit; files that are not initial conditions are refused with a message."""
@@ -29,6 +31,7 @@ HARNESS = r"""
 #include <cstdio>
 #include <exception>
 #include <fstream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -61,6 +64,22 @@ int main(int argc, char** argv) {
             std::vector<double> U(4 * n);
             for (size_t k = 0; k < n; ++k) put(U, n, k, 1.0 + 0.001 * k, 1.0, 1.0, 1.0);
             npde_io::write_snapshot(path, g, U, 0.5, {"sine2d", "hllc", "vanleer", 0.5, 149});
+        } else if (mode == "traj" && argc == 6) {  // lax_liu3 at 32^2 to t = 0.3, levels 32 and 16
+            // argv[3]: frame times then the levels' float32 frames, as raw bytes; argv[4]: frames; argv[5]: steps
+            const npde_grid_desc g{32, 32, 0.0, 1.0, 0.0, 1.0, NPDE_BC_TRANSMISSIVE, NPDE_BC_TRANSMISSIVE};
+            const int frames = std::stoi(argv[4]);
+            std::ifstream in(argv[3], std::ios::binary);
+            std::vector<double> times(frames);
+            in.read(reinterpret_cast<char*>(times.data()), std::streamsize(frames * sizeof(double)));
+            std::vector<npde_io::Level> levels;
+            for (int m : {32, 16}) {
+                npde_io::Level L{m, m, std::vector<float>(size_t(frames) * 4 * m * m)};
+                in.read(reinterpret_cast<char*>(L.data.data()), std::streamsize(L.data.size() * sizeof(float)));
+                levels.push_back(std::move(L));
+            }
+            if (!in) throw std::runtime_error("short frame file");
+            npde_io::write_trajectory(path, g, times, levels, {"lax_liu3", "hllc", "vanleer", 0.5, std::stoi(argv[5])},
+                                      0.3, 0.0);
         } else if (mode == "read" && argc == 4) {
             npde_grid_desc g{};
             std::vector<double> U;
@@ -147,6 +166,28 @@ def test_snapshot_layout_matches_reference_cli(harness, tmp_path):
     assert harness("sine32", path).returncode == 0
     got = subprocess.run([H5DUMP, "-p", "-A", str(path)], capture_output=True, text=True, check=True).stdout
     assert got.replace(str(path), path.name) == gold
+
+
+def test_trajectory_layout_matches_reference_cli(harness, oracle, tmp_path):
+    """The oracle's Lax-Liu 32^2 trajectory (frames at 32^2 and 16^2, which equal the reference CLI's bit for
+    bit) written by write_trajectory: h5dump -p -A (datasets, float32 chunks of one frame, shuffle + deflate 4,
+    the compressed sizes, attribute types and values) equals the reference CLI's --traj file
+    (trajectory.hpp:53-91), and the frames read back unchanged."""
+    gold = json.loads((GOLDEN / "h5_layout.json").read_text())["traj_ll32"]
+    g, t_end, U = oracle.setup("lax_liu3", 32)
+    out = oracle.run(g, "hllc", "vanleer", 0.5, U, t_end, nout=2, sizes=(32, 16))
+    raw = tmp_path / "frames.bin"
+    raw.write_bytes(out["frame_t"].astype("<f8").tobytes() + out["frames"][32].astype("<f4").tobytes()
+                    + out["frames"][16].astype("<f4").tobytes())
+    path = tmp_path / "ll32.h5"
+    r = harness("traj", path, raw, 3, out["steps"])
+    assert r.returncode == 0, r.stderr
+    got = subprocess.run([H5DUMP, "-p", "-A", str(path)], capture_output=True, text=True, check=True).stdout
+    assert got.replace(str(path), path.name) == gold
+    for m in (32, 16):
+        assert np.array_equal(h5(path, f"U_{m}", "<f4").reshape(3, 4, m, m).view(np.uint32),
+                              out["frames"][m].view(np.uint32)), m
+    assert np.array_equal(h5(path, "t", "<f8"), out["frame_t"])
 
 
 def test_snapshot_reads_back_as_an_initial_condition(harness, oracle, tmp_path):
