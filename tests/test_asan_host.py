@@ -3,6 +3,7 @@ by g++ with the sanitizer) driven by npde.dataset's own write paths in a child p
 import os
 import shutil
 import subprocess
+import sys
 from pathlib import Path
 
 import pytest
@@ -65,7 +66,7 @@ def test_packer_is_clean_under_address_sanitizer(tmp_path):
                     f"-Wl,-rpath,{hdf5_lib.parent}"], check=True)
     env = dict(os.environ, LD_PRELOAD=asan, NPDE_IO_LIB=str(lib),
                ASAN_OPTIONS="detect_leaks=0:halt_on_error=1")
-    r = subprocess.run([os.sys.executable, "-c", ASAN_SCRIPT, str(ROOT / "neural-pde-surrogate_amd")], env=env,
+    r = subprocess.run([sys.executable, "-c", ASAN_SCRIPT, str(ROOT / "neural-pde-surrogate_amd")], env=env,
                        capture_output=True, text=True, timeout=300)
     assert "AddressSanitizer" not in r.stderr, r.stderr[-4000:]
     assert r.returncode == 0 and "PACKER RUN COMPLETE" in r.stdout, r.stderr[-4000:]
