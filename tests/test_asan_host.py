@@ -1,5 +1,5 @@
-"""Host code of the dataset job under AddressSanitizer, on the CPU: the HDF5 packer (libnpde_io.so, built here
-by g++ with the sanitizer) driven by npde.dataset's own write paths in a child process."""
+"""Host code under sanitizers, on the CPU: the dataset job's HDF5 packer (libnpde_io.so) and the oracle, each
+built here by g++ with the sanitizers and driven through its Python callers in a child process."""
 import os
 import shutil
 import subprocess
@@ -70,3 +70,64 @@ def test_packer_is_clean_under_address_sanitizer(tmp_path):
                        capture_output=True, text=True, timeout=300)
     assert "AddressSanitizer" not in r.stderr, r.stderr[-4000:]
     assert r.returncode == 0 and "PACKER RUN COMPLETE" in r.stdout, r.stderr[-4000:]
+
+
+# The oracle's entry points on every built-in problem, scheme and limiter, on grids narrower than the stencil,
+# on point-wise arrays, and through its non-physical failure path.
+ORACLE_SCRIPT = r"""
+import sys
+from pathlib import Path
+import numpy as np
+sys.path.insert(0, sys.argv[1]); sys.path.insert(0, sys.argv[2])
+import oracle2d
+from npde.families import riemann
+orc = oracle2d.Oracle(Path(sys.argv[3]))
+for test, n in (("lax_liu3", 64), ("sod_x", 40), ("sod_y", 40), ("sine2d", 32), ("kh", 32)):
+    g, t_end, U = orc.setup(test, n)
+    for s in ("rusanov", "hllc"):
+        for l in ("none", "unlimited", "minmod", "vanleer"):
+            a = U.copy()
+            orc.run(g, s, l, 0.5, a, min(t_end, 0.05), nout=3, sizes=((n // 2,) if g.nx == g.ny else ()),
+                    record_dts=True)
+for nx, ny in ((1, 1), (1, 5), (5, 1), (2, 3), (7, 3)):
+    for bx in ("periodic", "transmissive"):
+        g = oracle2d.Grid.make(nx, ny, y=(0.0, ny / nx), bc_x=bx, bc_y="periodic")
+        U = np.ascontiguousarray(riemann(3, 8)[0][:, :ny, :nx])
+        orc.run(g, "hllc", "vanleer", 0.5, U, 0.02, nout=2, record_dts=True)
+        orc.rhs(g, "hllc", "vanleer", U)
+        orc.compute_dt(g, 0.5, U)
+W = np.random.default_rng(0).uniform(0.1, 2.0, (1000, 4))
+C = orc.to_conserved(W)
+orc.to_primitive(C); orc.sound_speed(W); orc.physical_flux(C)
+for l in ("none", "unlimited", "minmod", "vanleer"):
+    orc.limiter(l, W[:, 0] - 1.0, W[:, 1] - 1.0)
+for s in ("rusanov", "hllc"):
+    for d in (0, 1):
+        orc.face_flux(s, d, C, C[::-1].copy())
+g, t_end, U = orc.setup("sod_x", 100)
+assert orc.run(g, "rusanov", "unlimited", 0.5, U, 0.2, nout=1)["rc"] == 1
+print("ORACLE RUN COMPLETE")
+"""
+
+
+def test_oracle_is_clean_under_address_and_undefined_behaviour_sanitizers(tmp_path):
+    """The checker itself (oracle/euler2d_oracle.cpp) built by g++ with -fsanitize=address,undefined and driven
+    through oracle2d in a child process: no invalid access and no undefined behaviour."""
+    gxx = shutil.which("g++")
+    if not gxx:
+        pytest.skip("needs g++")
+    rt = [subprocess.run([gxx, f"-print-file-name={lib}"], capture_output=True, text=True).stdout.strip()
+          for lib in ("libasan.so", "libubsan.so")]
+    if not all(os.path.isabs(p) and os.path.exists(p) for p in rt):
+        pytest.skip("no sanitizer runtimes")
+    lib = tmp_path / "liboracle2d_san.so"
+    subprocess.run([gxx, "-O1", "-g", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
+                    "-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-shared", "-o", str(lib),
+                    str(ROOT / "oracle" / "euler2d_oracle.cpp")], check=True)
+    env = dict(os.environ, LD_PRELOAD=":".join(rt), ASAN_OPTIONS="detect_leaks=0:halt_on_error=1",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    r = subprocess.run([sys.executable, "-c", ORACLE_SCRIPT, str(ROOT / "oracle"),
+                        str(ROOT / "neural-pde-surrogate_amd"), str(lib)], env=env, capture_output=True, text=True,
+                       timeout=600)
+    assert "AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-4000:]
+    assert r.returncode == 0 and "ORACLE RUN COMPLETE" in r.stdout, r.stderr[-4000:]
