@@ -1,5 +1,6 @@
-"""Dataset pipeline (npde.dataset): split rule, statistics, the HDF5 packed-file writer, and (GPU) one
-small end-to-end run whose packed frames equal the solver's own trajectories."""
+"""Dataset pipeline (npde.dataset): split rule, statistics, the HDF5 packed-file writer, the reference's
+dataset families through the oracle against the reference's recorded step counts, and (GPU) one small
+end-to-end run whose packed frames equal the oracle's, and the recorded step counts on the GPU solver."""
 import json
 import os
 import shutil
@@ -105,6 +106,30 @@ def test_packed_files_are_deterministic(tmp_path):
         if k == 0:
             time.sleep(1.1)  # HDF5 timestamps have one-second resolution
     assert filecmp.cmp(tmp_path / "p0.h5", tmp_path / "p1.h5", shallow=False)
+
+
+def test_reference_families_reproduce_manifest_step_counts_on_the_oracle(oracle):
+    """The CPU half of the manifest pin: the reference's dataset families as restated in numpy
+    (npde.families.REFERENCE_FAMILIES, data/families.py) run through the oracle at the reference's 512^2 with
+    50 frames take the step counts the reference's euler2d recorded (tests/golden/reference_manifest_steps.json),
+    on the cheapest riemann and blast instances of the 400 each (912 and 440 steps); and every held-out KH
+    instance's drawn start time lies within one ulp of the manifest's.  A sampler that drew in another order,
+    from another seed or with another distribution would change the wave speeds and with them the counts."""
+    import oracle2d
+    from npde.families import REFERENCE_FAMILIES, ref_kh
+    rows = json.loads((ROOT / "tests" / "golden" / "reference_manifest_steps.json").read_text())["rows"]
+    by = {(r[0], r[1]): r for r in rows}
+    for fam in ("riemann", "blast"):
+        i = min(range(400), key=lambda k: by[(fam, k)][3])
+        U, ts, te, _ = REFERENCE_FAMILIES[fam](i, 512)
+        g = oracle2d.Grid.make(512, 512, bc_x="periodic", bc_y="periodic")
+        out = oracle.run(g, "hllc", "vanleer", 0.5, np.ascontiguousarray(U), te, t_start=ts, nout=50)
+        assert out["rc"] == 0 and out["steps"] == by[(fam, i)][3], (fam, i, out["steps"])
+    kh_rows = [r for r in rows if r[0] == "kh"]
+    assert len(kh_rows) == 50
+    for r in kh_rows:
+        t0 = ref_kh(r[1], 8)[1]
+        assert abs(t0 - r[2]) <= np.spacing(t0), r
 
 
 @pytest.mark.gpu
