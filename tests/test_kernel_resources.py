@@ -17,6 +17,7 @@ ROOT = Path(__file__).resolve().parents[1]
 LIB = ROOT / "neural-pde-surrogate_amd" / "lib" / "libnpde.so"
 LLVM = Path(os.environ.get("ROCM_PATH", "/opt/rocm")) / "llvm" / "bin"
 MAGIC = b"__CLANG_OFFLOAD_BUNDLE__"
+CODE_OBJECTS = []  # the extracted gfx950 code objects (filled by the kernels fixture)
 
 
 def waves_per_simd(vgprs: int) -> int:
@@ -45,8 +46,9 @@ def kernels(tmp_path_factory):
             pos += 24 + tlen
             if "gfx950" not in triple or size == 0:
                 continue
-            co = d / "k.co"
+            co = d / f"k{len(CODE_OBJECTS)}.co"
             co.write_bytes(data[start + off:start + off + size])
+            CODE_OBJECTS.append(co)
             notes = subprocess.run([str(readelf), "--notes", str(co)], capture_output=True, text=True,
                                    check=True).stdout
             for blk in notes.split("\n  - .")[1:]:
@@ -100,3 +102,20 @@ def test_network_and_layer_kernels_budgets(kernels):
     assert waves_per_simd(vg) == 2 and 2 * lds <= 160 * 1024
     conv = {k: v for k, v in kernels.items() if k.startswith("_ZN7npde_nn")}
     assert conv and all(2 * v[2] <= 160 * 1024 for v in conv.values())
+
+
+def test_no_packed_fp32_arithmetic(kernels):
+    """Every unit is compiled with the packed-FP32 target feature off (the Makefile's -packed-fp32-ops): no
+    kernel holds a packed FP32 add, multiply or FMA; FP32 work issues as single operations."""
+    objdump = LLVM / "llvm-objdump"
+    if not objdump.exists():
+        pytest.skip("no llvm-objdump")
+    assert CODE_OBJECTS
+    seen = set()
+    for co in CODE_OBJECTS:  # one per translation unit (the C-ABI's holds only the static staging array)
+        dis = subprocess.run([str(objdump), "-d", "--mcpu=gfx950", str(co)], capture_output=True, text=True,
+                             check=True).stdout
+        seen |= {op for op in ("v_add_f64", "v_mfma_f32_16x16x32_bf16", "v_fma_f64") if op in dis}
+        found = sorted(set(re.findall(r"\bv_pk_(?:add|mul|fma)_f32\b", dis)))
+        assert not found, (co.name, found)
+    assert seen == {"v_add_f64", "v_mfma_f32_16x16x32_bf16", "v_fma_f64"}  # the kernels were disassembled
