@@ -12,6 +12,12 @@ ROOT = Path(__file__).resolve().parents[1]
 IO_LIB = ROOT / "neural-pde-surrogate_amd" / "lib" / "libnpde_io.so"
 
 
+def preload(*runtimes):
+    """LD_PRELOAD for the child: the sanitizer runtimes first (they must load before anything else), then
+    whatever the environment already preloads."""
+    return ":".join([*runtimes, *filter(None, [os.environ.get("LD_PRELOAD", "")])])
+
+
 # The dataset job's host-side writes, run in a child process against a build of the writer instrumented by
 # AddressSanitizer: packed files of several shapes, converted (non-contiguous, other dtypes) inputs, rows
 # with and without a shared frame spacing, and the whole pack step (files, statistics, manifest) on
@@ -64,7 +70,7 @@ def test_packer_is_clean_under_address_sanitizer(tmp_path):
                     "-ffp-contract=off", "-DNPDE_WITH_HDF5", f"-I{hdf5_inc}", "-shared", "-fPIC", "-o", str(lib),
                     str(drv / "io_capi.cpp"), str(drv / "h5io.cpp"), str(hdf5_lib),
                     f"-Wl,-rpath,{hdf5_lib.parent}"], check=True)
-    env = dict(os.environ, LD_PRELOAD=asan, NPDE_IO_LIB=str(lib),
+    env = dict(os.environ, LD_PRELOAD=preload(asan), NPDE_IO_LIB=str(lib),
                ASAN_OPTIONS="detect_leaks=0:halt_on_error=1")
     r = subprocess.run([sys.executable, "-c", ASAN_SCRIPT, str(ROOT / "neural-pde-surrogate_amd")], env=env,
                        capture_output=True, text=True, timeout=300)
@@ -124,7 +130,7 @@ def test_oracle_is_clean_under_address_and_undefined_behaviour_sanitizers(tmp_pa
     subprocess.run([gxx, "-O1", "-g", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
                     "-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-shared", "-o", str(lib),
                     str(ROOT / "oracle" / "euler2d_oracle.cpp")], check=True)
-    env = dict(os.environ, LD_PRELOAD=":".join(rt), ASAN_OPTIONS="detect_leaks=0:halt_on_error=1",
+    env = dict(os.environ, LD_PRELOAD=preload(*rt), ASAN_OPTIONS="detect_leaks=0:halt_on_error=1",
                UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
     r = subprocess.run([sys.executable, "-c", ORACLE_SCRIPT, str(ROOT / "oracle"),
                         str(ROOT / "neural-pde-surrogate_amd"), str(lib)], env=env, capture_output=True, text=True,
