@@ -193,14 +193,17 @@ def test_benchmark_sizes_against_reference_build(oracle, reference):
     """The oracle equals the reference build at the benchmark's own sizes, so the GPU's bitwise checks against
     the oracle at those sizes reach the reference: Lax-Liu 256^2 to t = 0.3 with 10 frames (the reference's CPU
     case, 682 steps), a random-Riemann 512^2 instance of the benchmark batch over its first 0.02 of time, and an
-    instance of the held-out Kelvin-Helmholtz family at 256^2 over its first 0.05 -- every dt, the frame times
-    and the final state bit for bit."""
-    from npde.families import kh, riemann
+    instance of the held-out Kelvin-Helmholtz family at 256^2 over its first 0.05, and a 256^2 instance of the
+    uneven blast batch to its end time 0.2 (1,441 steps) -- every dt, the frame times and the final state bit
+    for bit."""
+    from npde.families import blast, blast_size, kh, riemann
     cases = []
     g, t_end, U = oracle.setup("lax_liu3", 256)
     cases.append((g, U, 0.0, t_end, 10))
     cases.append((oracle2d.Grid.make(512, 512, bc_x="periodic", bc_y="periodic"), riemann(0, 512)[0], 0.0, 0.02, 2))
     cases.append((oracle2d.Grid.make(256, 256, bc_x="periodic", bc_y="periodic"), kh(0, 256)[0], 0.0, 0.05, 2))
+    assert blast_size(5) == 256  # the uneven blast batch's first 256^2 instance, over its whole run to t = 0.2
+    cases.append((oracle2d.Grid.make(256, 256, bc_x="periodic", bc_y="periodic"), blast(5)[0], 0.0, 0.2, 2))
     for g, U0, t0, t1, nout in cases:
         a, b = np.ascontiguousarray(U0).copy(), np.ascontiguousarray(U0).copy()
         ra = oracle.run(g, "hllc", "vanleer", 0.5, a, t1, t_start=t0, nout=nout, record_dts=True)
